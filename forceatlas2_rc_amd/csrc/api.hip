@@ -1,0 +1,132 @@
+// C entry points of the all-GPU path: fa2_full_run (the drop-in of
+// src/gpu_full.c:164-532) and the batch API of fa2_hip.h.
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.h"
+#include "engine.h"
+#include "../../include/fa2.h"
+#include "../../include/fa2_hip.h"
+
+using namespace fa2;
+
+struct fa2_batch {
+    Engine *e = nullptr;
+    ~fa2_batch() { delete e; }
+};
+
+extern "C" fa2_batch *fa2_batch_open(int ngraphs, const fa2_full_params *params,
+                                     const double *const *pos)
+{
+    if (ngraphs < 1 || !params || !pos) {
+        set_message("no graph to lay out");
+        return nullptr;
+    }
+    const int device = params[0].device;
+    hipStream_t st = device_stream(device);
+    if (!st) return nullptr;
+    fa2_batch *b = new (std::nothrow) fa2_batch();
+    if (!b) {
+        set_message("out of memory");
+        return nullptr;
+    }
+    b->e = make_engine(params[0].use_double);
+    if (!b->e) {
+        delete b;
+        set_message("out of memory");
+        return nullptr;
+    }
+    b->e->device = device;
+    b->e->stream = st;
+    b->e->profile = params[0].profile;
+    b->e->interrupted = params[0].interrupted;
+    std::vector<EngineGraph> gr(ngraphs);
+    for (int g = 0; g < ngraphs; g++) {
+        gr[g].p = &params[g];
+        gr[g].pos = pos[g];
+    }
+    if (b->e->open(gr.data(), ngraphs)) {
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
+
+extern "C" int fa2_batch_run(fa2_batch *b, int iterations)
+{
+    if (!b) return 1;
+    return b->e->run(iterations);
+}
+
+extern "C" int fa2_batch_sync(fa2_batch *b)
+{
+    if (!b) return 1;
+    return b->e->sync();
+}
+
+extern "C" int fa2_batch_positions(fa2_batch *b, int g, double *pos)
+{
+    if (!b) return 1;
+    return b->e->positions(g, pos);
+}
+
+extern "C" int fa2_batch_forces(fa2_batch *b, int g, double *f)
+{
+    if (!b) return 1;
+    return b->e->forces(g, f);
+}
+
+extern "C" int fa2_batch_speed(fa2_batch *b, int g, double out[4])
+{
+    if (!b) return 1;
+    return b->e->speed_state(g, out);
+}
+
+extern "C" int fa2_batch_tree(fa2_batch *b, int g, uint32_t *keys_sorted, uint32_t *order,
+                              int *child, int *range, int *parent)
+{
+    if (!b) return 1;
+    return b->e->tree(g, keys_sorted, order, child, range, parent);
+}
+
+extern "C" int fa2_batch_stage_times(fa2_batch *b, double *times)
+{
+    if (!b) return 1;
+    return b->e->stage_times(times);
+}
+
+extern "C" int fa2_batch_count_work(fa2_batch *b, unsigned long long out[3])
+{
+    if (!b) return 1;
+    return b->e->count_work(out);
+}
+
+extern "C" void *fa2_batch_stream(fa2_batch *b) { return b ? (void *) b->e->stream : nullptr; }
+
+extern "C" void fa2_batch_close(fa2_batch *b) { delete b; }
+
+// fa2_full_run: upload, iterate, download (src/gpu_full.c:164-532). The
+// adaptive speed stays on the device; the host waits once, at the end.
+extern "C" int fa2_full_run(const fa2_full_params *p, double *pos, double *times)
+{
+    for (int i = 0; i < FA2_FULL_NTIMES; i++) times[i] = 0.0;
+    if (!p || p->n < 2 || p->n > (1 << 26)) {
+        set_message("the all-GPU layout handles between 2 and 67 million nodes");
+        return 1;
+    }
+    const double *pp = pos;
+    fa2_batch *b = fa2_batch_open(1, p, &pp);
+    if (!b) return 1;
+    const double start = fa2_now(); // the iterations and the download, as the reference
+    int status = 1;
+    if (b->e->run(p->iterations) == 0 && b->e->sync() == 0 && b->e->positions(0, pos) == 0) {
+        status = 0;
+        double t[FA2_FULL_NTIMES];
+        b->e->stage_times(t);
+        for (int i = 1; i < FA2_FULL_NTIMES; i++) times[i] = t[i];
+        times[FA2_FULL_T_TOTAL] = fa2_now() - start;
+    }
+    fa2_batch_close(b);
+    return status;
+}

@@ -1,0 +1,46 @@
+// Host interface of the all-GPU engine (engine.hip): one object per batch of
+// graphs resident on one device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/fa2.h"
+
+namespace fa2 {
+
+// One graph of a batch: its arguments (as for fa2_full_run) and its initial
+// positions (x[0..n) then y[0..n)).
+struct EngineGraph {
+    const fa2_full_params *p;
+    const double *pos;
+};
+
+struct Engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int profile = 0;
+    int (*interrupted)(void) = nullptr;
+    virtual ~Engine() = default;
+    // Upload the graphs; 0 on success (message in fa2_gpu_error()).
+    virtual int open(const EngineGraph *graphs, int ngraphs) = 0;
+    // Enqueue iterations; no host synchronisation.
+    virtual int run(int iterations) = 0;
+    // Wait for the device; 1 on a device error or invalid numbers.
+    virtual int sync() = 0;
+    virtual int positions(int graph, double *out) = 0;
+    virtual int forces(int graph, double *out) = 0;
+    virtual int speed_state(int graph, double out[4]) = 0;
+    virtual int tree(int graph, uint32_t *keys_sorted, uint32_t *order, int *child, int *range,
+                     int *parent) = 0;
+    virtual int stage_times(double *t) = 0;
+    virtual int graph_status(int graph) = 0;
+    // Work of the repulsion on the current tree: regions accepted, regions
+    // opened, node pairs (untimed instrumented traversal).
+    virtual int count_work(unsigned long long out[3]) = 0;
+};
+
+Engine *make_engine(int use_double);
+
+} // namespace fa2

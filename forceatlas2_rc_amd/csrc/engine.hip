@@ -1,0 +1,1448 @@
+// The all-GPU ForceAtlas2 engine of libfa2hip.so: one or many graphs laid
+// out side by side in HBM, every stage of every iteration on the device.
+//
+// It replaces the pipeline of the reference all-GPU layout
+// (inst/opencl/pipeline.cl, host loop src/gpu_full.c:164-532). A single
+// graph (fa2_full_run) is a batch of one; the batch entry points of
+// fa2_hip.h pack many graphs of uneven size into the same launches.
+//
+// Layout in HBM (N nodes of G graphs, graph g owning the node range
+// [noff_g, noff_g + n_g), the same range before and after sorting):
+//   pos    R2   positions used by the kernels (float32 copies in single
+//               precision, the positions themselves in double precision)
+//   acc    f64x2 position accumulators (single precision only)
+//   F[2]   R2   force of this iteration / of the previous one (ping-pong)
+//   keys/vals   Morton codes and node ids, two buffers for the radix sort
+//   psort  R4   (x, y, mass, radius) in sorted order
+//   tn     {R4 (cx, cy, mass, size^2), int4 (left, skip, first, last)} for
+//               the n_g - 1 regions of each graph, stored at
+//               [noff_g - g, noff_g - g + n_g - 1)
+//   CSR    inc_off (N+1), inc_node (I), inc_coef (I, R)
+// Node references inside a tree: r >= 0 a region, r < 0 the leaf at sorted
+// position ~r, kEnd the end of a traversal.
+//
+// Every reduction is made in a fixed order: results are reproducible run
+// to run, and a graph gives the same result alone and in a batch as long
+// as the tiles of its nodes do not change (they depend only on the graph).
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.h"
+#include "kernels_common.h"
+#include "engine.h"
+#include "../../include/fa2.h"
+
+namespace fa2 {
+
+constexpr int kEnd = INT_MIN;
+constexpr int kLeafSize = 8; // regions of at most 8 nodes are visited node by node
+
+// ---------------------------------------------------------------------------
+// Device descriptors
+// ---------------------------------------------------------------------------
+
+enum : int { GF_STRONG = 1, GF_LINLOG = 2, GF_ADJUST = 4, GF_FIXED = 8 };
+
+struct GraphDesc {
+    int noff, n, ioff, toff;
+    int ntiles, flags, iters, pad;
+    double scaling, gravity, theta2, jt;
+};
+
+struct GraphState {
+    double speed, eff, tsw, ttr;
+    int done;   // iterations applied
+    int status; // 0, or 2 when the totals were not numbers
+    int active; // 1 when this iteration moves the graph
+    int pad;
+};
+
+struct Tile {
+    int g, lo, hi, j; // graph, node range [lo, hi), tile index in the graph
+};
+
+template <typename R> struct TNode {
+    typename Vec<R>::v4 c; // centre x, centre y, mass, size^2
+    int4 l;                // left child, skip, first, last (sorted positions)
+};
+
+// ---------------------------------------------------------------------------
+// Codes: bounding boxes and Morton keys (pipeline.cl:61-130)
+// ---------------------------------------------------------------------------
+
+template <typename R>
+__device__ inline void block_minmax4(R &a, R &b, R &c, R &d, R *lds /* 4*4 */)
+{
+    a = wave_min(a);
+    b = wave_min(b);
+    c = wave_max(c);
+    d = wave_max(d);
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        lds[4 * w] = a;
+        lds[4 * w + 1] = b;
+        lds[4 * w + 2] = c;
+        lds[4 * w + 3] = d;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int) (blockDim.x >> 6); k++) {
+            a = min(a, lds[4 * k]);
+            b = min(b, lds[4 * k + 1]);
+            c = max(c, lds[4 * k + 2]);
+            d = max(d, lds[4 * k + 3]);
+        }
+    }
+}
+
+// Bounding box of each tile (used once before the first iteration; later
+// the displacement kernel produces it).
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_tile_bbox(const Tile *__restrict__ tiles, const typename Vec<R>::v2 *__restrict__ pos,
+            typename Vec<R>::v4 *__restrict__ tbox)
+{
+    __shared__ R lds[16];
+    const Tile t = tiles[blockIdx.x];
+    R a = (R) INFINITY, b = (R) INFINITY, c = (R) -INFINITY, d = (R) -INFINITY;
+    for (int i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
+        const typename Vec<R>::v2 p = pos[i];
+        a = min(a, p.x);
+        b = min(b, p.y);
+        c = max(c, p.x);
+        d = max(d, p.y);
+    }
+    block_minmax4<R>(a, b, c, d, lds);
+    if (threadIdx.x == 0) tbox[blockIdx.x] = mk4<R>(a, b, c, d);
+}
+
+// Bounding box of each graph from its tiles: one wave per graph.
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_graph_bbox(int G, const GraphDesc *__restrict__ gd, const typename Vec<R>::v4 *__restrict__ tbox,
+             typename Vec<R>::v4 *__restrict__ gbox)
+{
+    const int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (g >= G) return;
+    const int lane = lane_id();
+    const int t0 = gd[g].toff, nt = gd[g].ntiles;
+    R a = (R) INFINITY, b = (R) INFINITY, c = (R) -INFINITY, d = (R) -INFINITY;
+    for (int t = lane; t < nt; t += 64) {
+        const typename Vec<R>::v4 v = tbox[t0 + t];
+        a = min(a, v.x);
+        b = min(b, v.y);
+        c = max(c, v.z);
+        d = max(d, v.w);
+    }
+    a = wave_min(a);
+    b = wave_min(b);
+    c = wave_max(c);
+    d = wave_max(d);
+    if (lane == 0) gbox[g] = mk4<R>(a, b, c, d);
+}
+
+__device__ inline uint32_t spread16(uint32_t v)
+{
+    v &= 0x0000FFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+__device__ inline float div_rn(float a, float b) { return __fdiv_rn(a, b); }
+__device__ inline double div_rn(double a, double b) { return __ddiv_rn(a, b); }
+
+// Morton code of every node: 16 bits per axis, x in the even bits, computed
+// with IEEE arithmetic as written (the codes must match the reference bit for
+// bit, pipeline.cl:116-130).
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_morton(const Tile *__restrict__ tiles, const typename Vec<R>::v2 *__restrict__ pos,
+         const typename Vec<R>::v4 *__restrict__ gbox, uint32_t *__restrict__ key,
+         uint32_t *__restrict__ val)
+{
+    const Tile t = tiles[blockIdx.x];
+    const typename Vec<R>::v4 bb = gbox[t.g];
+    const R ex = bb.z - bb.x, ey = bb.w - bb.y;
+    const R extent = ex > ey ? ex : ey;
+    const R scale = extent > (R) 0 ? div_rn((R) 65535, extent) : (R) 0;
+    for (int i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
+        const typename Vec<R>::v2 p = pos[i];
+        R fx = (p.x - bb.x) * scale;
+        R fy = (p.y - bb.y) * scale;
+        fx = min(max(fx, (R) 0), (R) 65535);
+        fy = min(max(fy, (R) 0), (R) 65535);
+        key[i] = spread16((uint32_t) fx) | (spread16((uint32_t) fy) << 1);
+        val[i] = (uint32_t) i;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Segmented stable LSD radix sort, 4 passes of 8 bits (pipeline.cl:137-188).
+// Tiles never straddle two graphs; the scan is made per (graph, digit) row,
+// so every pass keeps each graph in its own node range.
+// hist row (g, d) starts at toff_g*256 + d*ntiles_g.
+// ---------------------------------------------------------------------------
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kTile = kSortThreads * kSortItems; // 2048 nodes
+
+__global__ void __launch_bounds__(kSortThreads)
+k_sort_upsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
+               const uint32_t *__restrict__ key, int shift, uint32_t *__restrict__ hist)
+{
+    __shared__ uint32_t h[256];
+    const Tile t = tiles[blockIdx.x];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int i = t.lo + threadIdx.x; i < t.hi; i += kSortThreads)
+        atomicAdd(&h[(key[i] >> shift) & 255u], 1u);
+    __syncthreads();
+    const GraphDesc g = gd[t.g];
+    hist[(size_t) g.toff * 256 + (size_t) threadIdx.x * g.ntiles + t.j] = h[threadIdx.x];
+}
+
+// One wave per (graph, digit) row: exclusive scan in place, row total out.
+__global__ void __launch_bounds__(256)
+k_sort_rowscan(int rows, const GraphDesc *__restrict__ gd, uint32_t *__restrict__ hist,
+               uint32_t *__restrict__ rowtot)
+{
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    const int g = r >> 8, d = r & 255;
+    const GraphDesc G = gd[g];
+    uint32_t *row = hist + (size_t) G.toff * 256 + (size_t) d * G.ntiles;
+    const int lane = lane_id();
+    uint32_t carry = 0;
+    for (int b = 0; b < G.ntiles; b += 64) {
+        const int j = b + lane;
+        const uint32_t v = j < G.ntiles ? row[j] : 0u;
+        uint32_t s = v; // inclusive scan across the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(s, o, 64);
+            if (lane >= o) s += u;
+        }
+        if (j < G.ntiles) row[j] = carry + s - v;
+        carry += __shfl(s, 63, 64);
+    }
+    if (lane == 0) rowtot[r] = carry;
+}
+
+// Stable scatter of one tile. Wave w ranks its 512 keys in 8 rounds of 64
+// (ballot match on the 8 digit bits), then the waves are combined in order:
+// the rank of a key counts the equal digits before it in the tile. Keys go
+// through LDS so that the global writes of one digit are contiguous.
+__global__ void __launch_bounds__(kSortThreads)
+k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
+                 const uint32_t *__restrict__ key_in, const uint32_t *__restrict__ val_in,
+                 int shift, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ rowtot,
+                 uint32_t *__restrict__ key_out, uint32_t *__restrict__ val_out)
+{
+    __shared__ uint32_t wcnt[4][256];
+    __shared__ int goff[256];
+    __shared__ uint32_t skey[kTile];
+    __shared__ uint32_t sval[kTile];
+    __shared__ uint32_t scan_tmp[256];
+
+    const Tile t = tiles[blockIdx.x];
+    const GraphDesc G = gd[t.g];
+    const int cnt = t.hi - t.lo;
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    const int tid = threadIdx.x;
+    for (int k = 0; k < 4; k++) wcnt[k][tid] = 0;
+    __syncthreads();
+
+    uint32_t kk[kSortItems], vv[kSortItems];
+    int dig[kSortItems], rank[kSortItems];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const int q = w * (kSortItems * 64) + r * 64 + lane;
+        const bool ok = q < cnt;
+        kk[r] = ok ? key_in[t.lo + q] : 0u;
+        vv[r] = ok ? val_in[t.lo + q] : 0u;
+        const int d = (int) ((kk[r] >> shift) & 255u);
+        dig[r] = ok ? d : -1;
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int before = __popcll(peers & lt);
+        // the lowest lane of each digit reserves the slots of its peers
+        uint32_t base = 0;
+        if (ok && before == 0) base = atomicAdd(&wcnt[w][d], (uint32_t) __popcll(peers));
+        const int leader = __ffsll((unsigned long long) peers) - 1;
+        base = __shfl(base, leader < 0 ? lane : leader, 64);
+        rank[r] = (int) base + before;
+    }
+    __syncthreads();
+
+    // per digit: counts of the waves, tile total, then the tile's digit starts
+    const uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
+    const uint32_t tot = c0 + c1 + c2 + c3;
+    // block exclusive scan of tot over the 256 digits
+    scan_tmp[tid] = tot;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t u = tid >= o ? scan_tmp[tid - o] : 0u;
+        __syncthreads();
+        scan_tmp[tid] += u;
+        __syncthreads();
+    }
+    const uint32_t start = scan_tmp[tid] - tot;
+    wcnt[0][tid] = start;
+    wcnt[1][tid] = start + c0;
+    wcnt[2][tid] = start + c0 + c1;
+    wcnt[3][tid] = start + c0 + c1 + c2;
+    // digit bases of the graph: exclusive scan of its 256 row totals
+    const uint32_t rt = rowtot[t.g * 256 + tid];
+    __syncthreads();
+    scan_tmp[tid] = rt;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t u = tid >= o ? scan_tmp[tid - o] : 0u;
+        __syncthreads();
+        scan_tmp[tid] += u;
+        __syncthreads();
+    }
+    const uint32_t gbase = scan_tmp[tid] - rt;
+    const uint32_t pre = hist[(size_t) G.toff * 256 + (size_t) tid * G.ntiles + t.j];
+    goff[tid] = (int) (G.noff + gbase + pre) - (int) start;
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        if (dig[r] < 0) continue;
+        const int p = (int) wcnt[w][dig[r]] + rank[r];
+        skey[p] = kk[r];
+        sval[p] = vv[r];
+    }
+    __syncthreads();
+    for (int p = tid; p < cnt; p += kSortThreads) {
+        const uint32_t k = skey[p];
+        const int d = (int) ((k >> shift) & 255u);
+        const int dst = goff[d] + p;
+        if (dst < G.noff || dst >= G.noff + G.n) continue; // never taken
+        key_out[dst] = k;
+        val_out[dst] = sval[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tree: gather, Karras radix tree, bottom-up summaries (pipeline.cl:195-323)
+// ---------------------------------------------------------------------------
+
+template <typename R>
+__global__ void k_gather(int N, const uint32_t *__restrict__ order,
+                         const typename Vec<R>::v2 *__restrict__ pos, const R *__restrict__ mass,
+                         const R *__restrict__ size, typename Vec<R>::v4 *__restrict__ psort)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const uint32_t i = order[k];
+    const typename Vec<R>::v2 p = pos[i];
+    psort[k] = mk4<R>(p.x, p.y, mass[i], size[i]);
+}
+
+// Common prefix of sorted keys a and b of one graph (local positions),
+// equal keys told apart by their positions; -1 out of range
+// (pipeline.cl:211-217).
+__device__ inline int kdelta(const uint32_t *__restrict__ ks, int n, int a, int b)
+{
+    if (b < 0 || b >= n) return -1;
+    const uint32_t x = ks[a], y = ks[b];
+    if (x == y) return 32 + __clz((int) ((uint32_t) a ^ (uint32_t) b));
+    return __clz((int) (x ^ y));
+}
+
+// Region t of graph gi[t] (Karras 2012, as pipeline.cl:226-270), with global
+// node references. Also records, for the gap after each sorted position, the
+// node a traversal visits next (the right child of the region split there),
+// and clears the arrival counters.
+template <typename R>
+__global__ void k_karras(int NI, const int *__restrict__ gi, const GraphDesc *__restrict__ gd,
+                         const uint32_t *__restrict__ ks, int2 *__restrict__ child,
+                         TNode<R> *__restrict__ tn, int *__restrict__ parent_leaf,
+                         int *__restrict__ parent_int, int *__restrict__ escape,
+                         int *__restrict__ visits, int *__restrict__ ready_count)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= NI) return;
+    const int g = gi[t];
+    const GraphDesc G = gd[g];
+    const int n = G.n, i = t - G.ioff, base = G.noff;
+    const uint32_t *k = ks + base;
+    const int d = kdelta(k, n, i, i + 1) > kdelta(k, n, i, i - 1) ? 1 : -1;
+    const int dmin = kdelta(k, n, i, i - d);
+    int lmax = 2;
+    while (kdelta(k, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int s = lmax / 2; s >= 1; s /= 2)
+        if (kdelta(k, n, i, i + (l + s) * d) > dmin) l += s;
+    const int j = i + l * d;
+    const int dnode = kdelta(k, n, i, j);
+    int s = 0, st = l;
+    do {
+        st = (st + 1) >> 1;
+        if (kdelta(k, n, i, i + (s + st) * d) > dnode) s += st;
+    } while (st > 1);
+    const int split = i + s * d + min(d, 0);
+    const int first = min(i, j), last = max(i, j);
+    const int S = base + split;
+    const int left = first == split ? ~S : S - g;
+    const int right = last == split + 1 ? ~(S + 1) : S + 1 - g;
+    child[t] = make_int2(left, right);
+    if (left >= 0) parent_int[left] = t;
+    else parent_leaf[~left] = t;
+    if (right >= 0) parent_int[right] = t;
+    else parent_leaf[~right] = t;
+    if (i == 0) parent_int[t] = -1;
+    escape[S - g] = right;
+    tn[t].l = make_int4(left, 0, base + first, base + last);
+    visits[t] = 0;
+    if (t == 0) *ready_count = 0;
+}
+
+template <typename R> struct Summ {
+    R x0, y0, x1, y1, m, mx, my;
+};
+
+template <typename R>
+__device__ inline Summ<R> leaf_summ(const typename Vec<R>::v4 p)
+{
+    return Summ<R>{p.x, p.y, p.x, p.y, p.z, p.x * p.z, p.y * p.z};
+}
+
+// Summaries of region t from those of its two children, merged left then
+// right (pipeline.cl:276-305), and its traversal record (pipeline.cl:312-323)
+// with its skip link. The centre of mass is kept inside the bounding box of
+// the region, as exact arithmetic guarantees: rounded, it can fall an ulp
+// away from a cluster of coincident nodes, which would then see their own
+// region as a far monopole.
+template <typename R>
+__device__ inline void merge_region(int t, int g, int gend,
+                                    const typename Vec<R>::v4 *__restrict__ psort,
+                                    const int2 *__restrict__ child,
+                                    const int *__restrict__ escape,
+                                    typename Vec<R>::v4 *nd, TNode<R> *tn)
+{
+    const int2 c = child[t];
+    Summ<R> a, b;
+    if (c.x < 0) {
+        a = leaf_summ<R>(psort[~c.x]);
+    } else {
+        const typename Vec<R>::v4 u = nd[2 * c.x], v = nd[2 * c.x + 1];
+        a = Summ<R>{u.x, u.y, u.z, u.w, v.x, v.y, v.z};
+    }
+    if (c.y < 0) {
+        b = leaf_summ<R>(psort[~c.y]);
+    } else {
+        const typename Vec<R>::v4 u = nd[2 * c.y], v = nd[2 * c.y + 1];
+        b = Summ<R>{u.x, u.y, u.z, u.w, v.x, v.y, v.z};
+    }
+    Summ<R> r;
+    r.x0 = min(a.x0, b.x0);
+    r.y0 = min(a.y0, b.y0);
+    r.x1 = max(a.x1, b.x1);
+    r.y1 = max(a.y1, b.y1);
+    r.m = a.m + b.m;
+    r.mx = a.mx + b.mx;
+    r.my = a.my + b.my;
+    nd[2 * t] = mk4<R>(r.x0, r.y0, r.x1, r.y1);
+    nd[2 * t + 1] = mk4<R>(r.m, r.mx, r.my, (R) 0);
+    const R w = r.x1 - r.x0, h = r.y1 - r.y0;
+    const R cx = min(max(r.mx / r.m, r.x0), r.x1);
+    const R cy = min(max(r.my / r.m, r.y0), r.y1);
+    tn[t].c = mk4<R>(cx, cy, r.m, (R) 2 * (w * w + h * h));
+    const int last = tn[t].l.w;
+    tn[t].l.y = last == gend - 1 ? kEnd : escape[last - g];
+}
+
+// Leaf to root, first inside each tile: the regions whose nodes all lie in
+// one tile are merged by that tile's workgroup, the second child to arrive
+// merging (arrival counters in LDS, workgroup-scope acquire/release). A
+// region spanning several tiles only counts the arrival in HBM; when both
+// of its children are complete it joins the ready list of k_aggregate_top.
+// No agent-scope fence is needed: the kernel boundary publishes the
+// summaries to the next kernel.
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
+                 const typename Vec<R>::v4 *__restrict__ psort, const int2 *__restrict__ child,
+                 const int *__restrict__ parent_leaf, const int *__restrict__ parent_int,
+                 const int *__restrict__ escape, int *gvis, int *ready, int *ready_count,
+                 typename Vec<R>::v4 *nd, TNode<R> *tn)
+{
+    __shared__ int lvis[kTile];
+    const Tile T = tiles[blockIdx.x];
+    const int g = T.g, lo = T.lo, hi = T.hi;
+    const int gend = gd[g].noff + gd[g].n;
+    const int tbase = lo - g; // regions inside the tile are numbered tbase..
+    for (int i = threadIdx.x; i < kTile; i += blockDim.x) lvis[i] = 0;
+    __syncthreads();
+    for (int k = lo + (int) threadIdx.x; k < hi; k += blockDim.x) {
+        int t = parent_leaf[k];
+        // a root-to-leaf path has at most 64 regions (32 key bits, then up to
+        // 32 bits of position); the bound only guards against a corrupt tree
+        for (int depth = 0; t >= 0 && depth < 128; depth++) {
+            const int4 l = tn[t].l;
+            if (l.z < lo || l.w >= hi) { // the region spans several tiles
+                if (atomicAdd(&gvis[t], 1) == 1) ready[atomicAdd(ready_count, 1)] = t;
+                break;
+            }
+            const int old = __hip_atomic_fetch_add(&lvis[t - tbase], 1, __ATOMIC_ACQ_REL,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0) break;
+            merge_region<R>(t, g, gend, psort, child, escape, nd, tn);
+            t = parent_int[t];
+        }
+    }
+}
+
+// The regions spanning several tiles (about tiles x depth of them), by one
+// workgroup: from the ready regions up, the second child to arrive merging.
+template <typename R>
+__global__ void __launch_bounds__(1024)
+k_aggregate_top(const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
+                const typename Vec<R>::v4 *__restrict__ psort, const int2 *__restrict__ child,
+                const int *__restrict__ parent_int, const int *__restrict__ escape,
+                const int *__restrict__ ready, const int *__restrict__ ready_count,
+                const int *__restrict__ gi, int *gvis, typename Vec<R>::v4 *nd, TNode<R> *tn)
+{
+    const int cnt = *ready_count;
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+        int t = ready[q];
+        const int g = gi[t];
+        const int gend = gd[g].noff + gd[g].n;
+        for (int depth = 0; depth < 128; depth++) {
+            merge_region<R>(t, g, gend, psort, child, escape, nd, tn);
+            const int p = parent_int[t];
+            if (p < 0) break;
+            const int old = __hip_atomic_fetch_add(&gvis[p], 1, __ATOMIC_ACQ_REL,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0) break;
+            t = p;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Repulsion: stackless Barnes-Hut over the Karras tree (pipeline.cl:332-383).
+// One lane per node in sorted (Morton) order, so the lanes of a wave are
+// neighbours in space and walk mostly the same regions. Regions are visited
+// in the reference's order (left child first), so each node accumulates the
+// same terms in the same order as the reference.
+// ---------------------------------------------------------------------------
+
+// With COUNT, the kernel also counts its work for the roofline: regions
+// accepted (A) and opened (O), and node pairs (P), into cnt[0..2].
+template <typename R, bool ANY_ADJUST, bool COUNT = false>
+__global__ void __launch_bounds__(256)
+k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
+          const typename Vec<R>::v4 *__restrict__ psort, const TNode<R> *__restrict__ tn,
+          const int *__restrict__ escape, const uint32_t *__restrict__ order,
+          typename Vec<R>::v2 *__restrict__ F, unsigned long long *cnt = nullptr)
+{
+    unsigned long long nA = 0, nO = 0, nP = 0;
+    using V4 = typename Vec<R>::v4;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int g = gid[k];
+    const GraphDesc G = gd[g];
+    const int gend = G.noff + G.n;
+    const bool adjust = ANY_ADJUST && (G.flags & GF_ADJUST);
+    const R theta2 = (R) G.theta2;
+    const V4 me = psort[k];
+    const R smi = (R) G.scaling * me.z;
+    R ax = (R) 0, ay = (R) 0;
+    int cur = G.n >= 2 ? G.ioff : ~k;
+    // every node of the tree is visited at most once (2n - 1 of them); the
+    // bound only guards against a corrupt tree
+    for (int steps = 2 * G.n; cur != kEnd && steps > 0; steps--) {
+        int j0, j1, nxt;
+        if (cur >= 0) {
+            const V4 c = tn[cur].c;
+            const int4 l = tn[cur].l;
+            const R xd = me.x - c.x, yd = me.y - c.y;
+            const R d2 = xd * xd + yd * yd;
+            if (d2 * theta2 > c.w) {
+                const R f = (smi * c.z) * frcp_r(d2);
+                ax += xd * f;
+                ay += yd * f;
+                cur = l.y;
+                if (COUNT) nA++;
+                continue;
+            }
+            if (COUNT) nO++;
+            if (l.w - l.z >= kLeafSize) {
+                cur = l.x;
+                continue;
+            }
+            j0 = l.z;
+            j1 = l.w;
+            nxt = l.y;
+        } else {
+            j0 = j1 = ~cur;
+            nxt = j0 == gend - 1 ? kEnd : escape[j0 - g];
+        }
+        for (int j = j0; j <= j1; j++) {
+            if (j == k) continue;
+            const V4 o = psort[j];
+            const R xd = me.x - o.x, yd = me.y - o.y;
+            const R d2 = xd * xd + yd * yd;
+            const R f = adjust ? pair_factor<R, true>(d2, smi * o.z, me.w + o.w)
+                               : pair_factor<R, false>(d2, smi * o.z, (R) 0);
+            ax += xd * f;
+            ay += yd * f;
+            if (COUNT) nP++;
+        }
+        cur = nxt;
+    }
+    F[order[k]] = mk2<R>(ax, ay);
+    if (COUNT) {
+        atomicAdd(&cnt[0], nA);
+        atomicAdd(&cnt[1], nO);
+        atomicAdd(&cnt[2], nP);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gravity and attraction (pipeline.cl:392-429), degree-binned: a node of the
+// class GL is handled by GL lanes that stride over its incidences; the lanes
+// are combined with a fixed butterfly.
+// ---------------------------------------------------------------------------
+
+template <typename R, int GL>
+__global__ void __launch_bounds__(256)
+k_attract(int count, const int *__restrict__ nodes, const int *__restrict__ gid,
+          const GraphDesc *__restrict__ gd, const typename Vec<R>::v2 *__restrict__ pos,
+          const R *__restrict__ mass, const R *__restrict__ size, const int *__restrict__ inc_off,
+          const int *__restrict__ inc_node, const R *__restrict__ inc_coef,
+          typename Vec<R>::v2 *__restrict__ F)
+{
+    __shared__ R red[2][4];
+    int slot, sub;
+    if (GL <= 64) {
+        const int gpb = 256 / GL; // groups per block
+        slot = blockIdx.x * gpb + (int) threadIdx.x / GL;
+        sub = (int) threadIdx.x % GL;
+    } else {
+        slot = blockIdx.x;
+        sub = threadIdx.x;
+    }
+    const bool live = slot < count;
+    const int i = live ? nodes[slot] : 0;
+    const int g = gid[i];
+    const GraphDesc G = gd[g];
+    const bool linlog = G.flags & GF_LINLOG, adjust = G.flags & GF_ADJUST;
+    const typename Vec<R>::v2 pi = pos[i];
+    const R si = size[i];
+    R ax = (R) 0, ay = (R) 0;
+    if (live) {
+        const int e1 = inc_off[i + 1];
+        for (int e = inc_off[i] + sub; e < e1; e += GL) {
+            const int j = inc_node[e];
+            const typename Vec<R>::v2 pj = pos[j];
+            const R xd = pi.x - pj.x, yd = pi.y - pj.y;
+            R f = -inc_coef[e];
+            if (adjust || linlog) {
+                R dist = fsqrt_r(xd * xd + yd * yd);
+                if (adjust) dist -= si + size[j];
+                if (!(dist > (R) 0)) continue;
+                if (linlog) f *= flog1p_r(dist) / dist;
+            }
+            ax += xd * f;
+            ay += yd * f;
+        }
+    }
+    if (GL > 1) {
+        const int width = GL < 64 ? GL : 64;
+#pragma unroll
+        for (int o = width / 2; o >= 1; o >>= 1) {
+            ax += __shfl_xor(ax, o, 64);
+            ay += __shfl_xor(ay, o, 64);
+        }
+        if (GL > 64) {
+            const int w = threadIdx.x >> 6;
+            if (lane_id() == 0) {
+                red[0][w] = ax;
+                red[1][w] = ay;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                ax = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+                ay = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
+            }
+        }
+    }
+    if (!live || sub != 0) return;
+    typename Vec<R>::v2 f = F[i];
+    const R grav = (R) G.gravity;
+    if (grav != (R) 0) {
+        const R dist = fsqrt_r(pi.x * pi.x + pi.y * pi.y);
+        if (dist > (R) 0) {
+            const R gg = (G.flags & GF_STRONG) ? grav * mass[i] : grav * mass[i] / dist;
+            f.x -= pi.x * gg;
+            f.y -= pi.y * gg;
+        }
+    }
+    f.x += ax;
+    f.y += ay;
+    F[i] = f;
+}
+
+// ---------------------------------------------------------------------------
+// Swinging, traction, adaptive speed and displacement
+// (pipeline.cl:438-506, src/forceatlas2.c:445-485). Per-node values and all
+// sums are FP64; each tile gives one partial, each graph sums its tiles in a
+// fixed order.
+// ---------------------------------------------------------------------------
+
+template <typename R>
+__device__ inline void swing_node(typename Vec<R>::v2 f, typename Vec<R>::v2 fo, double m,
+                                  double &sw, double &tr)
+{
+    const double sx = (double) fo.x - (double) f.x, sy = (double) fo.y - (double) f.y;
+    const double tx = (double) fo.x + (double) f.x, ty = (double) fo.y + (double) f.y;
+    sw = m * __builtin_sqrt(sx * sx + sy * sy);
+    tr = 0.5 * m * __builtin_sqrt(tx * tx + ty * ty);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_swing(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
+        const R *__restrict__ mass, const uint8_t *__restrict__ fixed,
+        const typename Vec<R>::v2 *__restrict__ F, const typename Vec<R>::v2 *__restrict__ Fold,
+        double2 *__restrict__ part)
+{
+    __shared__ double red[2][4];
+    const Tile t = tiles[blockIdx.x];
+    const bool use_fixed = gd[t.g].flags & GF_FIXED;
+    double a = 0.0, b = 0.0;
+    for (int i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
+        double sw, tr;
+        swing_node<R>(F[i], Fold[i], (double) mass[i], sw, tr);
+        if (use_fixed && fixed[i]) continue;
+        a += sw;
+        b += tr;
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        red[0][w] = a;
+        red[1][w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = make_double2(((red[0][0] + red[0][1]) + red[0][2]) + red[0][3],
+                                        ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3]);
+}
+
+// Device restatement of fa2_update_speed (src/forceatlas2.c:445-485).
+__device__ inline void update_speed(int n, double sw, double tr, double jt_user, double &speed,
+                                    double &eff)
+{
+    if (!(sw > 0)) return;
+    const double est = 0.05 * __builtin_sqrt((double) n);
+    const double lo = __builtin_sqrt(est), hi = 10.0;
+    double jt = est * tr / ((double) n * (double) n);
+    if (jt > hi) jt = hi;
+    if (jt < lo) jt = lo;
+    jt *= jt_user;
+    if (sw > 2.0 * tr) {
+        if (eff > 0.05) eff *= 0.5;
+        if (jt < jt_user) jt = jt_user;
+    }
+    const double target = jt * eff * tr / sw;
+    if (sw > jt * tr) {
+        if (eff > 0.05) eff *= 0.7;
+    } else if (speed < 1000) {
+        eff *= 1.3;
+    }
+    const double rise = target - speed;
+    speed += rise < 0.5 * speed ? rise : 0.5 * speed;
+}
+
+// One wave per graph: totals of its tiles, then the new speed.
+__global__ void __launch_bounds__(256)
+k_speed(int G, const GraphDesc *__restrict__ gd, const double2 *__restrict__ part,
+        GraphState *__restrict__ gs)
+{
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= G) return;
+    const int lane = lane_id();
+    const GraphDesc D = gd[g];
+    double a = 0.0, b = 0.0;
+    for (int t = lane; t < D.ntiles; t += 64) {
+        const double2 v = part[D.toff + t];
+        a += v.x;
+        b += v.y;
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if (lane != 0) return;
+    GraphState s = gs[g];
+    s.active = 0;
+    if (s.status == 0 && s.done < D.iters) {
+        if (!(a == a) || !(b == b)) {
+            s.status = 2; // the totals are not numbers (src/gpu_full.c:486-491)
+        } else {
+            update_speed(D.n, a, b, D.jt, s.speed, s.eff);
+            s.tsw = a;
+            s.ttr = b;
+            s.done += 1;
+            s.active = 1;
+        }
+    }
+    gs[g] = s;
+}
+
+template <typename R> struct Acc;
+template <> struct Acc<float> {
+    static constexpr bool separate = true;
+};
+template <> struct Acc<double> {
+    static constexpr bool separate = false;
+};
+
+// Displacement (pipeline.cl:480-506) into the FP64 accumulators, float32
+// copies for the next iteration, and the tile's bounding box.
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_apply(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
+        const GraphState *__restrict__ gs, const R *__restrict__ mass,
+        const uint8_t *__restrict__ fixed, const typename Vec<R>::v2 *__restrict__ F,
+        const typename Vec<R>::v2 *__restrict__ Fold, typename Vec<R>::v2 *__restrict__ pos,
+        double2 *__restrict__ acc, typename Vec<R>::v4 *__restrict__ tbox)
+{
+    __shared__ R lds[16];
+    const Tile t = tiles[blockIdx.x];
+    const GraphDesc D = gd[t.g];
+    const GraphState S = gs[t.g];
+    const bool move = S.active;
+    const bool use_fixed = D.flags & GF_FIXED, adjust = D.flags & GF_ADJUST;
+    const double speed = S.speed;
+    R a = (R) INFINITY, b = (R) INFINITY, c = (R) -INFINITY, d = (R) -INFINITY;
+    for (int i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
+        typename Vec<R>::v2 p = pos[i];
+        if (move && !(use_fixed && fixed[i])) {
+            const typename Vec<R>::v2 f = F[i];
+            double sw, tr;
+            swing_node<R>(f, Fold[i], (double) mass[i], sw, tr);
+            double factor;
+            if (adjust) {
+                const double fx = (double) f.x, fy = (double) f.y;
+                const double df = __builtin_sqrt(fx * fx + fy * fy);
+                factor = 0.1 * speed / (1.0 + __builtin_sqrt(speed * sw));
+                if (df > 0) {
+                    if (factor * df > 10.0) factor = 10.0 / df;
+                } else {
+                    factor = 0.0;
+                }
+            } else {
+                factor = speed / (1.0 + __builtin_sqrt(speed * sw));
+            }
+            if (Acc<R>::separate) {
+                double2 q = acc[i];
+                q.x += (double) f.x * factor;
+                q.y += (double) f.y * factor;
+                acc[i] = q;
+                p = mk2<R>((R) q.x, (R) q.y);
+            } else {
+                p.x += (R) ((double) f.x * factor);
+                p.y += (R) ((double) f.y * factor);
+            }
+            pos[i] = p;
+        }
+        a = min(a, p.x);
+        b = min(b, p.y);
+        c = max(c, p.x);
+        d = max(d, p.y);
+    }
+    block_minmax4<R>(a, b, c, d, lds);
+    if (threadIdx.x == 0) tbox[blockIdx.x] = mk4<R>(a, b, c, d);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+template <typename R> struct EngineImpl final : Engine {
+    using V2 = typename Vec<R>::v2;
+    using V4 = typename Vec<R>::v4;
+
+    int G = 0, N = 0, NI = 0, T = 0;
+    long long I = 0;
+    std::vector<GraphDesc> h_gd;
+    std::vector<Tile> h_tiles;
+    std::vector<int> h_class[4];
+    bool any_adjust = false;
+    int parity = 0; // F[parity] receives the force of the next iteration
+
+    DevBuf<GraphDesc> d_gd;
+    DevBuf<GraphState> d_gs;
+    DevBuf<Tile> d_tiles;
+    DevBuf<int> d_gid, d_gi;
+    DevBuf<V2> d_pos, d_F[2];
+    DevBuf<double2> d_acc;
+    DevBuf<R> d_mass, d_size, d_coef;
+    DevBuf<uint8_t> d_fixed;
+    DevBuf<int> d_inc_off, d_inc_node;
+    DevBuf<int> d_class[4];
+    DevBuf<V4> d_tbox, d_gbox, d_psort, d_nd;
+    DevBuf<uint32_t> d_keyA, d_valA, d_keyB, d_valB, d_hist, d_rowtot;
+    DevBuf<int2> d_child;
+    DevBuf<TNode<R>> d_tn;
+    DevBuf<int> d_parent_leaf, d_parent_int, d_escape, d_visits;
+    DevBuf<int> d_ready; // regions spanning tiles whose children are done, then their count
+    DevBuf<double2> d_part;
+
+    // profiling: events between the phases of each iteration
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::vector<hipEvent_t>> ev_iters;
+    double stage_s[FA2_FULL_NTIMES] = {0};
+
+    ~EngineImpl() override
+    {
+        if (stream) (void) hipStreamSynchronize(stream);
+        for (hipEvent_t e : ev_pool) (void) hipEventDestroy(e);
+    }
+
+    int open(const EngineGraph *gr, int ngraphs) override;
+    int run(int iterations) override;
+    int sync() override;
+    int positions(int graph, double *out) override;
+    int forces(int graph, double *out) override;
+    int speed_state(int graph, double out[4]) override;
+    int tree(int graph, uint32_t *keys_sorted, uint32_t *order, int *child, int *range,
+             int *parent) override;
+    int stage_times(double *t) override;
+    int graph_status(int graph) override;
+    int count_work(unsigned long long out[3]) override;
+
+    void enqueue_iteration(std::vector<hipEvent_t> *ev);
+};
+
+template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs)
+{
+    G = ngraphs;
+    if (G < 1) {
+        set_message("no graph to lay out");
+        return 1;
+    }
+    // graph descriptors, tiles
+    long long nsum = 0, isum = 0;
+    for (int g = 0; g < G; g++) {
+        const fa2_full_params *p = gr[g].p;
+        if (p->n < 2 || p->n > (1 << 26)) {
+            set_message("the all-GPU layout handles between 2 and 67 million nodes");
+            return 1;
+        }
+        nsum += p->n;
+        isum += p->inc_off[p->n];
+    }
+    if (nsum > INT_MAX / 2 || isum > INT_MAX - 1) {
+        set_message("too many nodes or edges for one batch");
+        return 1;
+    }
+    N = (int) nsum;
+    I = isum;
+    NI = N - G;
+    h_gd.resize(G);
+    int noff = 0, toff = 0;
+    for (int g = 0; g < G; g++) {
+        const fa2_full_params *p = gr[g].p;
+        GraphDesc &D = h_gd[g];
+        memset(&D, 0, sizeof(D));
+        D.noff = noff;
+        D.n = p->n;
+        D.ioff = noff - g;
+        D.toff = toff;
+        D.ntiles = (int) ceil_div(p->n, kTile);
+        D.flags = (p->strong ? GF_STRONG : 0) | (p->linlog ? GF_LINLOG : 0) |
+                  (p->adjust ? GF_ADJUST : 0) | (p->any_fixed ? GF_FIXED : 0);
+        D.iters = p->iterations;
+        D.scaling = p->scaling;
+        D.gravity = p->gravity;
+        D.theta2 = p->theta * p->theta;
+        D.jt = p->jitter_tolerance;
+        any_adjust = any_adjust || p->adjust;
+        for (int j = 0; j < D.ntiles; j++) {
+            Tile t;
+            t.g = g;
+            t.lo = noff + j * kTile;
+            t.hi = std::min(noff + p->n, t.lo + kTile);
+            t.j = j;
+            h_tiles.push_back(t);
+        }
+        noff += p->n;
+        toff += D.ntiles;
+    }
+    T = toff;
+
+    // packed host arrays
+    const size_t nn = (size_t) N;
+    std::vector<int> gid(nn), gi(NI > 0 ? NI : 1), inc_off(nn + 1);
+    std::vector<int> inc_node(I > 0 ? I : 1);
+    std::vector<R> coef(I > 0 ? I : 1), mass(nn), size(nn);
+    std::vector<V2> pos(nn);
+    std::vector<double2> acc(Acc<R>::separate ? nn : 1);
+    std::vector<uint8_t> fixed(nn);
+    long long ioff = 0;
+    for (int g = 0; g < G; g++) {
+        const fa2_full_params *p = gr[g].p;
+        const GraphDesc &D = h_gd[g];
+        const double *x = gr[g].pos, *y = gr[g].pos + p->n;
+        for (int i = 0; i < p->n; i++) {
+            const size_t q = (size_t) D.noff + i;
+            gid[q] = g;
+            mass[q] = (R) p->mass[i];
+            size[q] = (p->adjust && p->size) ? (R) p->size[i] : (R) 0;
+            fixed[q] = (p->any_fixed && p->fixed) ? (p->fixed[i] != 0) : 0;
+            pos[q] = mk2<R>((R) x[i], (R) y[i]);
+            if (Acc<R>::separate) acc[q] = make_double2(x[i], y[i]);
+            inc_off[q] = (int) (ioff + p->inc_off[i]);
+        }
+        for (int i = 0; i < D.n - 1; i++) gi[(size_t) D.ioff + i] = g;
+        const int ni = p->inc_off[p->n];
+        for (int e = 0; e < ni; e++) {
+            const int j = p->inc_node[e];
+            if (j < 0 || j >= p->n) {
+                set_message("edge refers to a node that does not exist");
+                return 1;
+            }
+            inc_node[(size_t) ioff + e] = D.noff + j;
+            coef[(size_t) ioff + e] = (R) p->inc_coef[e];
+        }
+        ioff += ni;
+    }
+    inc_off[nn] = (int) ioff;
+    // degree classes of the attraction: 1, 16, 64 or 256 lanes per node
+    for (int i = 0; i < N; i++) {
+        const int deg = inc_off[i + 1] - inc_off[i];
+        const int c = deg <= 16 ? 0 : deg <= 256 ? 1 : deg <= 4096 ? 2 : 3;
+        h_class[c].push_back(i);
+    }
+
+    HipStatus hs;
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        if (hs.ok() && bytes) hs.check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream), "upload");
+    };
+    hs.check(d_gd.alloc(G), "alloc");
+    hs.check(d_gs.alloc(G), "alloc");
+    hs.check(d_tiles.alloc(T), "alloc");
+    hs.check(d_gid.alloc(nn), "alloc");
+    hs.check(d_gi.alloc(NI), "alloc");
+    hs.check(d_pos.alloc(nn), "alloc");
+    hs.check(d_F[0].alloc(nn), "alloc");
+    hs.check(d_F[1].alloc(nn), "alloc");
+    hs.check(d_acc.alloc(Acc<R>::separate ? nn : 1), "alloc");
+    hs.check(d_mass.alloc(nn), "alloc");
+    hs.check(d_size.alloc(nn), "alloc");
+    hs.check(d_fixed.alloc(nn), "alloc");
+    hs.check(d_inc_off.alloc(nn + 1), "alloc");
+    hs.check(d_inc_node.alloc((size_t) I), "alloc");
+    hs.check(d_coef.alloc((size_t) I), "alloc");
+    for (int c = 0; c < 4; c++) hs.check(d_class[c].alloc(h_class[c].size()), "alloc");
+    hs.check(d_tbox.alloc(T), "alloc");
+    hs.check(d_gbox.alloc(G), "alloc");
+    hs.check(d_psort.alloc(nn), "alloc");
+    hs.check(d_nd.alloc(2 * (size_t) (NI > 0 ? NI : 1)), "alloc");
+    hs.check(d_keyA.alloc(nn), "alloc");
+    hs.check(d_valA.alloc(nn), "alloc");
+    hs.check(d_keyB.alloc(nn), "alloc");
+    hs.check(d_valB.alloc(nn), "alloc");
+    hs.check(d_hist.alloc(256 * (size_t) T), "alloc");
+    hs.check(d_rowtot.alloc(256 * (size_t) G), "alloc");
+    hs.check(d_child.alloc(NI), "alloc");
+    hs.check(d_tn.alloc(NI), "alloc");
+    hs.check(d_parent_leaf.alloc(nn), "alloc");
+    hs.check(d_parent_int.alloc(NI), "alloc");
+    hs.check(d_escape.alloc(NI), "alloc");
+    hs.check(d_visits.alloc(NI), "alloc");
+    hs.check(d_ready.alloc((size_t) NI + 1), "alloc");
+    hs.check(d_part.alloc(T), "alloc");
+    if (!hs.ok()) {
+        set_hip_error("could not allocate memory on the device", hs.err);
+        return 1;
+    }
+    std::vector<GraphState> gs(G);
+    for (auto &s : gs) {
+        memset(&s, 0, sizeof(s));
+        s.speed = 1.0;
+        s.eff = 1.0;
+    }
+    up(d_gd.p, h_gd.data(), G * sizeof(GraphDesc));
+    up(d_gs.p, gs.data(), G * sizeof(GraphState));
+    up(d_tiles.p, h_tiles.data(), T * sizeof(Tile));
+    up(d_gid.p, gid.data(), nn * sizeof(int));
+    if (NI > 0) up(d_gi.p, gi.data(), (size_t) NI * sizeof(int));
+    up(d_pos.p, pos.data(), nn * sizeof(V2));
+    if (Acc<R>::separate) up(d_acc.p, acc.data(), nn * sizeof(double2));
+    up(d_mass.p, mass.data(), nn * sizeof(R));
+    up(d_size.p, size.data(), nn * sizeof(R));
+    up(d_fixed.p, fixed.data(), nn);
+    up(d_inc_off.p, inc_off.data(), (nn + 1) * sizeof(int));
+    up(d_inc_node.p, inc_node.data(), (size_t) I * sizeof(int));
+    up(d_coef.p, coef.data(), (size_t) I * sizeof(R));
+    for (int c = 0; c < 4; c++) up(d_class[c].p, h_class[c].data(), h_class[c].size() * sizeof(int));
+    if (hs.ok()) hs.check(hipMemsetAsync(d_F[0].p, 0, nn * sizeof(V2), stream), "memset");
+    if (hs.ok()) hs.check(hipMemsetAsync(d_F[1].p, 0, nn * sizeof(V2), stream), "memset");
+    if (hs.ok()) {
+        k_tile_bbox<R><<<T, 256, 0, stream>>>(d_tiles.p, d_pos.p, d_tbox.p);
+        hs.check(hipGetLastError(), "launch");
+    }
+    // the host vectors are freed on return: wait for the copies
+    if (hs.ok()) hs.check(hipStreamSynchronize(stream), "upload");
+    if (!hs.ok()) {
+        set_hip_error("could not send the graph to the device", hs.err);
+        return 1;
+    }
+    parity = 0;
+    return 0;
+}
+
+template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent_t> *ev)
+{
+    auto mark = [&]() {
+        if (!ev) return;
+        hipEvent_t e;
+        if (ev_pool.empty()) {
+            (void) hipEventCreate(&e);
+        } else {
+            e = ev_pool.back();
+            ev_pool.pop_back();
+        }
+        (void) hipEventRecord(e, stream);
+        ev->push_back(e);
+    };
+    V2 *Fc = d_F[parity].p, *Fo = d_F[1 - parity].p;
+    mark();
+    // 1-2: codes
+    k_graph_bbox<R><<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_tbox.p, d_gbox.p);
+    k_morton<R><<<T, 256, 0, stream>>>(d_tiles.p, d_pos.p, d_gbox.p, d_keyA.p, d_valA.p);
+    mark();
+    // 3: sort, A -> B -> A -> B -> A
+    for (int pass = 0; pass < 4; pass++) {
+        uint32_t *ki = pass % 2 ? d_keyB.p : d_keyA.p, *vi = pass % 2 ? d_valB.p : d_valA.p;
+        uint32_t *ko = pass % 2 ? d_keyA.p : d_keyB.p, *vo = pass % 2 ? d_valA.p : d_valB.p;
+        k_sort_upsweep<<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, ki, 8 * pass, d_hist.p);
+        k_sort_rowscan<<<ceil_div(256 * (size_t) G, 4), 256, 0, stream>>>(256 * G, d_gd.p, d_hist.p,
+                                                                          d_rowtot.p);
+        k_sort_downsweep<<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, ki, vi, 8 * pass,
+                                                         d_hist.p, d_rowtot.p, ko, vo);
+    }
+    mark();
+    // 4: tree
+    k_gather<R><<<ceil_div(N, 256), 256, 0, stream>>>(N, d_valA.p, d_pos.p, d_mass.p, d_size.p,
+                                                      d_psort.p);
+    if (NI > 0) {
+        k_karras<R><<<ceil_div(NI, 256), 256, 0, stream>>>(
+            NI, d_gi.p, d_gd.p, d_keyA.p, d_child.p, d_tn.p, d_parent_leaf.p, d_parent_int.p,
+            d_escape.p, d_visits.p, d_ready.p + NI);
+        k_aggregate_tile<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_psort.p, d_child.p,
+                                                    d_parent_leaf.p, d_parent_int.p, d_escape.p,
+                                                    d_visits.p, d_ready.p, d_ready.p + NI,
+                                                    d_nd.p, d_tn.p);
+        if (T > G) // some graph has several tiles
+            k_aggregate_top<R><<<1, 1024, 0, stream>>>(d_gid.p, d_gd.p, d_psort.p, d_child.p,
+                                                        d_parent_int.p, d_escape.p, d_ready.p,
+                                                        d_ready.p + NI, d_gi.p, d_visits.p,
+                                                        d_nd.p, d_tn.p);
+    }
+    mark();
+    // 5: repulsion
+    if (any_adjust)
+        k_repulse<R, true><<<ceil_div(N, 256), 256, 0, stream>>>(
+            N, d_gid.p, d_gd.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fc);
+    else
+        k_repulse<R, false><<<ceil_div(N, 256), 256, 0, stream>>>(
+            N, d_gid.p, d_gd.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fc);
+    mark();
+    // 6: gravity and attraction
+    {
+        const int c0 = (int) h_class[0].size(), c1 = (int) h_class[1].size();
+        const int c2 = (int) h_class[2].size(), c3 = (int) h_class[3].size();
+        if (c0)
+            k_attract<R, 1><<<ceil_div(c0, 256), 256, 0, stream>>>(
+                c0, d_class[0].p, d_gid.p, d_gd.p, d_pos.p, d_mass.p, d_size.p, d_inc_off.p,
+                d_inc_node.p, d_coef.p, Fc);
+        if (c1)
+            k_attract<R, 16><<<ceil_div(c1, 16), 256, 0, stream>>>(
+                c1, d_class[1].p, d_gid.p, d_gd.p, d_pos.p, d_mass.p, d_size.p, d_inc_off.p,
+                d_inc_node.p, d_coef.p, Fc);
+        if (c2)
+            k_attract<R, 64><<<ceil_div(c2, 4), 256, 0, stream>>>(
+                c2, d_class[2].p, d_gid.p, d_gd.p, d_pos.p, d_mass.p, d_size.p, d_inc_off.p,
+                d_inc_node.p, d_coef.p, Fc);
+        if (c3)
+            k_attract<R, 256><<<c3, 256, 0, stream>>>(c3, d_class[3].p, d_gid.p, d_gd.p,
+                                                       d_pos.p, d_mass.p, d_size.p, d_inc_off.p,
+                                                       d_inc_node.p, d_coef.p, Fc);
+    }
+    mark();
+    // 7-8: swinging, speed, displacement
+    k_swing<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_mass.p, d_fixed.p, Fc, Fo, d_part.p);
+    k_speed<<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_part.p, d_gs.p);
+    k_apply<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, d_mass.p, d_fixed.p, Fc, Fo,
+                                      d_pos.p, d_acc.p, d_tbox.p);
+    mark();
+    parity = 1 - parity;
+}
+
+template <typename R> int EngineImpl<R>::run(int iterations)
+{
+    (void) hipSetDevice(device);
+    double work = 0.0;
+    for (int it = 0; it < iterations; it++) {
+        work += (double) N + (double) I;
+        if (work >= 1e6) {
+            work = 0.0;
+            if (interrupted && interrupted()) {
+                set_message("the layout was interrupted");
+                return 1;
+            }
+        }
+        if (profile) {
+            ev_iters.emplace_back();
+            enqueue_iteration(&ev_iters.back());
+        } else {
+            enqueue_iteration(nullptr);
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            set_hip_error("the computation on the GPU failed", e);
+            return 1;
+        }
+    }
+    return 0;
+}
+
+template <typename R> int EngineImpl<R>::sync()
+{
+    (void) hipSetDevice(device);
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        set_hip_error("the computation on the GPU failed", e);
+        return 1;
+    }
+    // fold the profiling events into the stage times
+    static const int slot[6] = {FA2_FULL_T_CODES, FA2_FULL_T_SORT, FA2_FULL_T_TREE,
+                                FA2_FULL_T_REPULSION, FA2_FULL_T_ATTRACTION, FA2_FULL_T_MOVE};
+    for (auto &evs : ev_iters) {
+        for (size_t k = 0; k + 1 < evs.size() && k < 6; k++) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, evs[k], evs[k + 1]) == hipSuccess)
+                stage_s[slot[k]] += 1e-3 * ms;
+        }
+        for (hipEvent_t ev : evs) ev_pool.push_back(ev);
+    }
+    ev_iters.clear();
+    std::vector<GraphState> gs(G);
+    e = hipMemcpy(gs.data(), d_gs.p, G * sizeof(GraphState), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        set_hip_error("the computation on the GPU failed", e);
+        return 1;
+    }
+    for (int g = 0; g < G; g++)
+        if (gs[g].status) {
+            set_message("the computation on the GPU produced invalid numbers");
+            return 1;
+        }
+    return 0;
+}
+
+template <typename R> int EngineImpl<R>::graph_status(int graph)
+{
+    if (graph < 0 || graph >= G) return -1;
+    GraphState s;
+    if (hipMemcpy(&s, d_gs.p + graph, sizeof(s), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return s.status;
+}
+
+template <typename R> int EngineImpl<R>::positions(int graph, double *out)
+{
+    if (graph < 0 || graph >= G) {
+        set_message("no graph %d in this batch", graph);
+        return 1;
+    }
+    (void) hipSetDevice(device);
+    const GraphDesc &D = h_gd[graph];
+    const size_t n = (size_t) D.n;
+    hipError_t e;
+    if (Acc<R>::separate) {
+        std::vector<double2> h(n);
+        e = hipMemcpyAsync(h.data(), d_acc.p + D.noff, n * sizeof(double2), hipMemcpyDeviceToHost,
+                           stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        for (size_t i = 0; i < n && e == hipSuccess; i++) {
+            out[i] = h[i].x;
+            out[n + i] = h[i].y;
+        }
+    } else {
+        std::vector<V2> h(n);
+        e = hipMemcpyAsync(h.data(), d_pos.p + D.noff, n * sizeof(V2), hipMemcpyDeviceToHost,
+                           stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        for (size_t i = 0; i < n && e == hipSuccess; i++) {
+            out[i] = (double) h[i].x;
+            out[n + i] = (double) h[i].y;
+        }
+    }
+    if (e != hipSuccess) {
+        set_hip_error("could not fetch the layout from the device", e);
+        return 1;
+    }
+    return 0;
+}
+
+template <typename R> int EngineImpl<R>::forces(int graph, double *out)
+{
+    if (graph < 0 || graph >= G) {
+        set_message("no graph %d in this batch", graph);
+        return 1;
+    }
+    (void) hipSetDevice(device);
+    const GraphDesc &D = h_gd[graph];
+    const size_t n = (size_t) D.n;
+    std::vector<V2> h(n);
+    // the last force is in F[1 - parity]
+    hipError_t e = hipMemcpyAsync(h.data(), d_F[1 - parity].p + D.noff, n * sizeof(V2),
+                                  hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        set_hip_error("could not fetch the forces from the device", e);
+        return 1;
+    }
+    for (size_t i = 0; i < n; i++) {
+        out[i] = (double) h[i].x;
+        out[n + i] = (double) h[i].y;
+    }
+    return 0;
+}
+
+template <typename R> int EngineImpl<R>::speed_state(int graph, double out[4])
+{
+    if (graph < 0 || graph >= G) {
+        set_message("no graph %d in this batch", graph);
+        return 1;
+    }
+    GraphState s;
+    hipError_t e = hipMemcpy(&s, d_gs.p + graph, sizeof(s), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        set_hip_error("could not fetch the speed from the device", e);
+        return 1;
+    }
+    out[0] = s.tsw;
+    out[1] = s.ttr;
+    out[2] = s.speed;
+    out[3] = s.eff;
+    return 0;
+}
+
+template <typename R>
+int EngineImpl<R>::tree(int graph, uint32_t *keys_sorted, uint32_t *order, int *child, int *range,
+                        int *parent)
+{
+    if (graph < 0 || graph >= G) {
+        set_message("no graph %d in this batch", graph);
+        return 1;
+    }
+    (void) hipSetDevice(device);
+    const GraphDesc &D = h_gd[graph];
+    const int n = D.n;
+    std::vector<uint32_t> ks(n), ord(n);
+    std::vector<int2> ch(n - 1);
+    std::vector<TNode<R>> tn(n - 1);
+    std::vector<int> pl(n), pi(n - 1);
+    hipError_t e = hipStreamSynchronize(stream);
+    auto down = [&](void *dst, const void *src, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    };
+    down(ks.data(), d_keyA.p + D.noff, n * sizeof(uint32_t));
+    down(ord.data(), d_valA.p + D.noff, n * sizeof(uint32_t));
+    down(ch.data(), d_child.p + D.ioff, (n - 1) * sizeof(int2));
+    down(tn.data(), d_tn.p + D.ioff, (n - 1) * sizeof(TNode<R>));
+    down(pl.data(), d_parent_leaf.p + D.noff, n * sizeof(int));
+    down(pi.data(), d_parent_int.p + D.ioff, (n - 1) * sizeof(int));
+    if (e != hipSuccess) {
+        set_hip_error("could not fetch the tree from the device", e);
+        return 1;
+    }
+    // reference numbering: regions 0..n-2, leaves n-1..2n-2 (pipeline.cl:25-27)
+    auto ref = [&](int r) { return r >= 0 ? r - D.ioff : n - 1 + (~r - D.noff); };
+    for (int k = 0; k < n; k++) {
+        if (keys_sorted) keys_sorted[k] = ks[k];
+        if (order) order[k] = ord[k] - (uint32_t) D.noff;
+    }
+    for (int r = 0; r < n - 1; r++) {
+        if (child) {
+            child[2 * r] = ref(ch[r].x);
+            child[2 * r + 1] = ref(ch[r].y);
+        }
+        if (range) {
+            range[2 * r] = tn[r].l.z - D.noff;
+            range[2 * r + 1] = tn[r].l.w - D.noff;
+        }
+        if (parent) parent[r] = pi[r] < 0 ? -1 : pi[r] - D.ioff;
+    }
+    if (parent)
+        for (int k = 0; k < n; k++) parent[n - 1 + k] = pl[k] - D.ioff;
+    return 0;
+}
+
+template <typename R> int EngineImpl<R>::stage_times(double *t)
+{
+    for (int k = 0; k < FA2_FULL_NTIMES; k++) t[k] = stage_s[k];
+    return 0;
+}
+
+template <typename R> int EngineImpl<R>::count_work(unsigned long long out[3])
+{
+    (void) hipSetDevice(device);
+    DevBuf<unsigned long long> c;
+    hipError_t e = c.alloc(3);
+    if (e == hipSuccess) e = hipMemsetAsync(c.p, 0, 3 * sizeof(unsigned long long), stream);
+    // the tree of the last iteration; the forces go to the buffer that the
+    // next iteration overwrites first
+    V2 *Fs = d_F[parity].p;
+    if (e == hipSuccess) {
+        if (any_adjust)
+            k_repulse<R, true, true><<<ceil_div(N, 256), 256, 0, stream>>>(
+                N, d_gid.p, d_gd.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fs, c.p);
+        else
+            k_repulse<R, false, true><<<ceil_div(N, 256), 256, 0, stream>>>(
+                N, d_gid.p, d_gd.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fs, c.p);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, c.p, 3 * sizeof(unsigned long long),
+                                            hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        set_hip_error("could not count the work of the repulsion", e);
+        return 1;
+    }
+    return 0;
+}
+
+Engine *make_engine(int use_double)
+{
+    if (use_double) return new (std::nothrow) EngineImpl<double>();
+    return new (std::nothrow) EngineImpl<float>();
+}
+
+} // namespace fa2

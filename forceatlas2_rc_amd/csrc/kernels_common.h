@@ -1,0 +1,96 @@
+// Device helpers shared by the kernels of libfa2hip.so (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fa2 {
+
+template <typename R> struct Vec;
+template <> struct Vec<float> {
+    using v2 = float2;
+    using v4 = float4;
+};
+template <> struct Vec<double> {
+    using v2 = double2;
+    using v4 = double4;
+};
+
+template <typename R> __host__ __device__ inline typename Vec<R>::v2 mk2(R a, R b)
+{
+    typename Vec<R>::v2 v;
+    v.x = a;
+    v.y = b;
+    return v;
+}
+
+template <typename R> __host__ __device__ inline typename Vec<R>::v4 mk4(R a, R b, R c, R d)
+{
+    typename Vec<R>::v4 v;
+    v.x = a;
+    v.y = b;
+    v.z = c;
+    v.w = d;
+    return v;
+}
+
+// Reciprocal for the bulk force arithmetic: v_rcp_f32 (1 ulp) in FP32, IEEE
+// division in FP64.
+__device__ inline float frcp_r(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double frcp_r(double x) { return 1.0 / x; }
+
+// Square root for the bulk arithmetic: v_sqrt_f32 (1 ulp) in FP32, IEEE in
+// FP64.
+__device__ inline float fsqrt_r(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline double fsqrt_r(double x) { return __builtin_sqrt(x); }
+
+__device__ inline float flog1p_r(float x) { return log1pf(x); }
+__device__ inline double flog1p_r(double x) { return log1p(x); }
+
+// Repulsion factor between two nodes (reference rep_factor,
+// inst/opencl/pipeline.cl:46-55); smm = scaling * m_i * m_j, ss = s_i + s_j.
+template <typename R, bool ADJUST> __device__ inline R pair_factor(R d2, R smm, R ss)
+{
+    if (ADJUST) {
+        const R d = fsqrt_r(d2) - ss;
+        if (d > (R) 0) return smm * frcp_r(d * d);
+        if (d < (R) 0) return (R) 100 * smm;
+        return (R) 0;
+    }
+    return d2 > (R) 0 ? smm * frcp_r(d2) : (R) 0;
+}
+
+// Lane index inside the wave64.
+__device__ inline int lane_id() { return (int) __lane_id(); }
+
+// Wave-wide sums in a fixed order (xor butterfly): every lane ends with a
+// value; lane 0's is the one used, so the result is reproducible.
+template <typename T> __device__ inline T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename T> __device__ inline T wave_min(T v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const T w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+template <typename T> __device__ inline T wave_max(T v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const T w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+} // namespace fa2

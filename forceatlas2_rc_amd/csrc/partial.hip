@@ -1,0 +1,446 @@
+// Repulsion offload sessions of libfa2hip.so: fa2_gpu_open / _close /
+// _times / _repulse_exact / _repulse_bh, the backend of forceatlas2_gpu()
+// (reference: src/gpu.c:357-704, kernels src/gpu.c:28-123).
+//
+// Design for CDNA4:
+//  * The host arrays (FP64) are uploaded as they are and converted on the
+//    device, so no host threads are needed for the conversion.
+//  * Exact repulsion: every block owns 2x256 target nodes and one slice of
+//    the source nodes; sources are staged through LDS in tiles of 512 nodes
+//    and read back as broadcasts. Target blocks x source slices fill the 256
+//    CUs even for small n. The per-slice partial forces are summed in a
+//    fixed order, in FP64, by a second kernel: results are reproducible.
+//  * The bulk arithmetic is FP32 with v_rcp_f32 for the 1/d^2 (1 ulp); in
+//    double-precision sessions it is FP64 with IEEE division.
+//  * Barnes-Hut: one lane per node in tree order, stackless traversal along
+//    the skip links of the host tree, regions packed as two 16-byte records.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "common.h"
+#include "kernels_common.h"
+#include "../../include/fa2.h"
+
+using namespace fa2;
+
+namespace {
+
+constexpr int kExactThreads = 256;
+constexpr int kExactTargets = 2;    // targets per thread
+constexpr int kExactTile = 512;     // sources per LDS tile
+constexpr int kBhThreads = 256;
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// p[i] = (x, y, mass, size) in the session precision.
+template <typename R>
+__global__ void k_pack_nodes(int n, const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ m, const double *__restrict__ s,
+                             typename Vec<R>::v4 *__restrict__ p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    typename Vec<R>::v4 v;
+    v.x = (R) x[i];
+    v.y = (R) y[i];
+    v.z = (R) m[i];
+    v.w = s ? (R) s[i] : (R) 0;
+    p[i] = v;
+}
+
+// Exact repulsion of targets [blockIdx.x*512, +512) from the sources of
+// slice blockIdx.y; partial[slice][i] receives the sum (x, y).
+template <typename R, bool ADJUST>
+__global__ void __launch_bounds__(kExactThreads)
+k_exact(int n, int slice_len, R scaling, const typename Vec<R>::v4 *__restrict__ p,
+        typename Vec<R>::v2 *__restrict__ partial)
+{
+    using V4 = typename Vec<R>::v4;
+    __shared__ V4 tile[kExactTile];
+    const int i0 = blockIdx.x * (kExactThreads * kExactTargets) + threadIdx.x;
+    R xi[kExactTargets], yi[kExactTargets], mi[kExactTargets], si[kExactTargets];
+    R ax[kExactTargets], ay[kExactTargets];
+#pragma unroll
+    for (int t = 0; t < kExactTargets; t++) {
+        const int i = i0 + t * kExactThreads;
+        V4 v = i < n ? p[i] : V4{(R) 0, (R) 0, (R) 0, (R) 0};
+        xi[t] = v.x;
+        yi[t] = v.y;
+        mi[t] = scaling * v.z; // scaling * m_i, multiplied by m_j per pair
+        si[t] = v.w;
+        ax[t] = (R) 0;
+        ay[t] = (R) 0;
+    }
+    const int j_begin = blockIdx.y * slice_len;
+    const int j_end = min(n, j_begin + slice_len);
+    for (int j0 = j_begin; j0 < j_end; j0 += kExactTile) {
+        const int cnt = min(kExactTile, j_end - j0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < kExactTile; t += kExactThreads)
+            tile[t] = t < cnt ? p[j0 + t] : V4{(R) 0, (R) 0, (R) 0, (R) 0};
+        __syncthreads();
+        // Padding sources have mass 0: their contribution is exactly 0.
+#pragma unroll 4
+        for (int jj = 0; jj < kExactTile; jj++) {
+            const V4 s = tile[jj];
+#pragma unroll
+            for (int t = 0; t < kExactTargets; t++) {
+                const R xd = xi[t] - s.x, yd = yi[t] - s.y;
+                const R d2 = xd * xd + yd * yd;
+                R f;
+                if (ADJUST) {
+                    const R d = fsqrt_r(d2) - (si[t] + s.w);
+                    f = d > (R) 0 ? (mi[t] * s.z) * frcp_r(d * d)
+                                  : (d < (R) 0 ? (R) 100 * (mi[t] * s.z) : (R) 0);
+                } else {
+                    f = d2 > (R) 0 ? (mi[t] * s.z) * frcp_r(d2) : (R) 0;
+                }
+                ax[t] += xd * f;
+                ay[t] += yd * f;
+            }
+        }
+    }
+    typename Vec<R>::v2 *out = partial + (size_t) blockIdx.y * n;
+#pragma unroll
+    for (int t = 0; t < kExactTargets; t++) {
+        const int i = i0 + t * kExactThreads;
+        if (i < n) out[i] = typename Vec<R>::v2{ax[t], ay[t]};
+    }
+}
+
+// f[i] = sum over slices, in slice order, in FP64.
+template <typename R>
+__global__ void k_sum_slices(int n, int slices, const typename Vec<R>::v2 *__restrict__ partial,
+                             double2 *__restrict__ f)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sx = 0.0, sy = 0.0;
+    for (int s = 0; s < slices; s++) {
+        const typename Vec<R>::v2 v = partial[(size_t) s * n + i];
+        sx += (double) v.x;
+        sy += (double) v.y;
+    }
+    f[i] = make_double2(sx, sy);
+}
+
+// Regions of the host tree: rr = (cx, cy, mass, size^2), ri = (start, count,
+// last child or -1, skip) (the reference layout of src/gpu.c:501-519).
+template <typename R>
+__global__ void k_pack_regions(int nreg, const fa2_region *__restrict__ reg,
+                               typename Vec<R>::v4 *__restrict__ rr, int4 *__restrict__ ri)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nreg) return;
+    const fa2_region g = reg[r];
+    typename Vec<R>::v4 a;
+    a.x = (R) g.cx;
+    a.y = (R) g.cy;
+    a.z = (R) g.mass;
+    a.w = (R) (g.size * g.size);
+    rr[r] = a;
+    ri[r] = make_int4(g.start, g.count, g.nchild > 0 ? g.child + g.nchild - 1 : -1, g.skip);
+}
+
+// Barnes-Hut over the host tree (reference kernel fa2_bh, src/gpu.c:82-123):
+// the same regions in the same order as the CPU stack traversal.
+template <typename R, bool ADJUST>
+__global__ void __launch_bounds__(kBhThreads)
+k_bh_host_tree(int n, R scaling, R theta2, const typename Vec<R>::v4 *__restrict__ p,
+               const typename Vec<R>::v4 *__restrict__ rr, const int4 *__restrict__ ri,
+               typename Vec<R>::v2 *__restrict__ f)
+{
+    using V4 = typename Vec<R>::v4;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const V4 me = p[k];
+    const R smi = scaling * me.z;
+    R ax = (R) 0, ay = (R) 0;
+    int cur = 0;
+    while (cur >= 0) {
+        const int4 info = ri[cur];
+        if (info.y > 1) {
+            const V4 c = rr[cur];
+            const R xd = me.x - c.x, yd = me.y - c.y;
+            const R d2 = xd * xd + yd * yd;
+            if (d2 * theta2 > c.w) {
+                const R g = (smi * c.z) * frcp_r(d2);
+                ax += xd * g;
+                ay += yd * g;
+                cur = info.w;
+                continue;
+            }
+            if (info.z >= 0) {
+                cur = info.z;
+                continue;
+            }
+        }
+        for (int j = info.x; j < info.x + info.y; j++) {
+            if (j == k) continue;
+            const V4 o = p[j];
+            const R xd = me.x - o.x, yd = me.y - o.y;
+            const R d2 = xd * xd + yd * yd;
+            const R g = pair_factor<R, ADJUST>(d2, smi * o.z, me.w + o.w);
+            ax += xd * g;
+            ay += yd * g;
+        }
+        cur = info.w;
+    }
+    f[k] = typename Vec<R>::v2{ax, ay};
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Sessions
+// ---------------------------------------------------------------------------
+
+struct fa2_gpu {
+    int device, n, use_double, adjust, barnes_hut, threads;
+    hipStream_t stream;
+    double t_send, t_compute, t_fetch;
+    int slices, slice_len;
+    DevBuf<double> d_in;       // x, y, mass, size (4n) or px, py, pm, ps
+    DevBuf<unsigned char> d_p; // packed nodes, float4 or double4
+    DevBuf<unsigned char> d_partial;
+    DevBuf<double2> d_f64;
+    DevBuf<unsigned char> d_f; // BH forces in tree order
+    DevBuf<fa2_region> d_reg;
+    DevBuf<unsigned char> d_rr;
+    DevBuf<int4> d_ri;
+    PinBuf<double> h_in;
+    PinBuf<double2> h_f64;
+    PinBuf<unsigned char> h_f;
+    PinBuf<fa2_region> h_reg;
+};
+
+extern "C" void fa2_gpu_close(fa2_gpu *g)
+{
+    if (!g) return;
+    if (g->stream) (void) hipStreamSynchronize(g->stream);
+    delete g;
+}
+
+extern "C" void fa2_gpu_times(const fa2_gpu *g, double times[3])
+{
+    times[0] = g->t_send;
+    times[1] = g->t_compute;
+    times[2] = g->t_fetch;
+}
+
+extern "C" fa2_gpu *fa2_gpu_open(int device, int use_double, int n, int adjust,
+                                 int barnes_hut, int threads)
+{
+    if (device < 0 || device >= device_count()) {
+        set_message("there is no GPU device %d", device + 1);
+        return nullptr;
+    }
+    if (n < 1 || n > (1 << 27)) {
+        set_message("the GPU backend handles at most %d nodes", 1 << 27);
+        return nullptr;
+    }
+    hipStream_t st = device_stream(device);
+    if (!st) return nullptr;
+    fa2_gpu *g = new (std::nothrow) fa2_gpu();
+    if (!g) {
+        set_message("out of memory");
+        return nullptr;
+    }
+    g->device = device;
+    g->n = n;
+    g->use_double = use_double ? 1 : 0;
+    g->adjust = adjust ? 1 : 0;
+    g->barnes_hut = barnes_hut ? 1 : 0;
+    g->threads = threads > 1 ? threads : 1;
+    g->stream = st;
+    const size_t nn = (size_t) n;
+    const size_t rs = g->use_double ? sizeof(double) : sizeof(float);
+
+    // Exact: target blocks x source slices, at least ~2048 blocks in all.
+    const int tblocks = (int) ceil_div(nn, kExactThreads * kExactTargets);
+    int slices = (int) ceil_div(2048, tblocks);
+    if (slices > 64) slices = 64;
+    int slice_len = (int) ceil_div(nn, slices);
+    slice_len = (int) (ceil_div(slice_len, kExactTile) * kExactTile);
+    slices = (int) ceil_div(nn, slice_len);
+    g->slices = slices;
+    g->slice_len = slice_len;
+
+    HipStatus hs;
+    hs.check(g->d_in.alloc(4 * nn), "alloc");
+    hs.check(g->d_p.alloc(4 * nn * rs), "alloc");
+    hs.check(g->h_in.alloc(4 * nn), "alloc");
+    if (g->barnes_hut) {
+        hs.check(g->d_f.alloc(2 * nn * rs), "alloc");
+        hs.check(g->h_f.alloc(2 * nn * rs), "alloc");
+        hs.check(g->d_reg.alloc(2 * nn), "alloc");
+        hs.check(g->h_reg.alloc(2 * nn), "alloc");
+        hs.check(g->d_rr.alloc(2 * nn * 4 * rs), "alloc");
+        hs.check(g->d_ri.alloc(2 * nn), "alloc");
+    } else {
+        hs.check(g->d_partial.alloc((size_t) slices * nn * 2 * rs), "alloc");
+        hs.check(g->d_f64.alloc(nn), "alloc");
+        hs.check(g->h_f64.alloc(nn), "alloc");
+    }
+    if (!hs.ok()) {
+        set_hip_error("could not allocate memory on the device", hs.err);
+        fa2_gpu_close(g);
+        return nullptr;
+    }
+    return g;
+}
+
+template <typename R>
+static hipError_t launch_exact(fa2_gpu *g, double scaling)
+{
+    using V4 = typename Vec<R>::v4;
+    using V2 = typename Vec<R>::v2;
+    const int n = g->n;
+    const double *in = g->d_in.p;
+    k_pack_nodes<R><<<ceil_div(n, 256), 256, 0, g->stream>>>(
+        n, in, in + n, in + 2 * (size_t) n, g->adjust ? in + 3 * (size_t) n : nullptr,
+        (V4 *) g->d_p.p);
+    dim3 grid(ceil_div(n, kExactThreads * kExactTargets), g->slices);
+    if (g->adjust)
+        k_exact<R, true><<<grid, kExactThreads, 0, g->stream>>>(
+            n, g->slice_len, (R) scaling, (const V4 *) g->d_p.p, (V2 *) g->d_partial.p);
+    else
+        k_exact<R, false><<<grid, kExactThreads, 0, g->stream>>>(
+            n, g->slice_len, (R) scaling, (const V4 *) g->d_p.p, (V2 *) g->d_partial.p);
+    k_sum_slices<R><<<ceil_div(n, 256), 256, 0, g->stream>>>(
+        n, g->slices, (const V2 *) g->d_partial.p, g->d_f64.p);
+    return hipGetLastError();
+}
+
+extern "C" int fa2_gpu_repulse_exact(fa2_gpu *g, const double *x, const double *y,
+                                     const double *mass, const double *size,
+                                     double scaling, double *dx, double *dy)
+{
+    if (!g || g->barnes_hut) {
+        set_message("this GPU session does not compute the exact repulsion");
+        return 1;
+    }
+    const size_t n = (size_t) g->n;
+    HipStatus hs;
+    double t0 = fa2_now(), t1;
+    (void) hipSetDevice(g->device);
+    memcpy(g->h_in.p, x, n * sizeof(double));
+    memcpy(g->h_in.p + n, y, n * sizeof(double));
+    memcpy(g->h_in.p + 2 * n, mass, n * sizeof(double));
+    if (g->adjust && size) memcpy(g->h_in.p + 3 * n, size, n * sizeof(double));
+    else if (g->adjust) memset(g->h_in.p + 3 * n, 0, n * sizeof(double));
+    hs.check(hipMemcpyAsync(g->d_in.p, g->h_in.p, 4 * n * sizeof(double),
+                            hipMemcpyHostToDevice, g->stream), "upload");
+    t1 = fa2_now();
+    g->t_send += t1 - t0;
+    t0 = t1;
+    if (hs.ok()) hs.check(g->use_double ? launch_exact<double>(g, scaling)
+                                        : launch_exact<float>(g, scaling), "launch");
+    if (hs.ok()) hs.check(hipStreamSynchronize(g->stream), "compute");
+    t1 = fa2_now();
+    g->t_compute += t1 - t0;
+    t0 = t1;
+    if (hs.ok()) hs.check(hipMemcpyAsync(g->h_f64.p, g->d_f64.p, n * sizeof(double2),
+                                         hipMemcpyDeviceToHost, g->stream), "fetch");
+    if (hs.ok()) hs.check(hipStreamSynchronize(g->stream), "fetch");
+    if (!hs.ok()) {
+        set_hip_error("the computation on the GPU failed", hs.err);
+        return 1;
+    }
+    for (size_t i = 0; i < n; i++) {
+        dx[i] += g->h_f64.p[i].x;
+        dy[i] += g->h_f64.p[i].y;
+    }
+    g->t_fetch += fa2_now() - t0;
+    return 0;
+}
+
+template <typename R>
+static hipError_t launch_bh(fa2_gpu *g, int nreg, double scaling, double theta)
+{
+    using V4 = typename Vec<R>::v4;
+    using V2 = typename Vec<R>::v2;
+    const int n = g->n;
+    const double *in = g->d_in.p;
+    k_pack_nodes<R><<<ceil_div(n, 256), 256, 0, g->stream>>>(
+        n, in, in + n, in + 2 * (size_t) n, g->adjust ? in + 3 * (size_t) n : nullptr,
+        (V4 *) g->d_p.p);
+    k_pack_regions<R><<<ceil_div(nreg, 256), 256, 0, g->stream>>>(
+        nreg, g->d_reg.p, (V4 *) g->d_rr.p, g->d_ri.p);
+    const R th2 = (R) (theta * theta);
+    if (g->adjust)
+        k_bh_host_tree<R, true><<<ceil_div(n, kBhThreads), kBhThreads, 0, g->stream>>>(
+            n, (R) scaling, th2, (const V4 *) g->d_p.p, (const V4 *) g->d_rr.p, g->d_ri.p,
+            (V2 *) g->d_f.p);
+    else
+        k_bh_host_tree<R, false><<<ceil_div(n, kBhThreads), kBhThreads, 0, g->stream>>>(
+            n, (R) scaling, th2, (const V4 *) g->d_p.p, (const V4 *) g->d_rr.p, g->d_ri.p,
+            (V2 *) g->d_f.p);
+    return hipGetLastError();
+}
+
+extern "C" int fa2_gpu_repulse_bh(fa2_gpu *g, const fa2_tree *t, double scaling,
+                                  double theta, double *dx, double *dy)
+{
+    if (!g || !g->barnes_hut) {
+        set_message("this GPU session does not compute the Barnes-Hut repulsion");
+        return 1;
+    }
+    const size_t n = (size_t) g->n;
+    const int nreg = t->nreg;
+    if (nreg < 1 || (size_t) nreg > 2 * n) {
+        set_message("invalid Barnes-Hut tree (%d regions for %d nodes)", nreg, g->n);
+        return 1;
+    }
+    HipStatus hs;
+    double t0 = fa2_now(), t1;
+    (void) hipSetDevice(g->device);
+    memcpy(g->h_in.p, t->px, n * sizeof(double));
+    memcpy(g->h_in.p + n, t->py, n * sizeof(double));
+    memcpy(g->h_in.p + 2 * n, t->pm, n * sizeof(double));
+    if (g->adjust && t->ps) memcpy(g->h_in.p + 3 * n, t->ps, n * sizeof(double));
+    else if (g->adjust) memset(g->h_in.p + 3 * n, 0, n * sizeof(double));
+    memcpy(g->h_reg.p, t->reg, (size_t) nreg * sizeof(fa2_region));
+    hs.check(hipMemcpyAsync(g->d_in.p, g->h_in.p, 4 * n * sizeof(double),
+                            hipMemcpyHostToDevice, g->stream), "upload");
+    hs.check(hipMemcpyAsync(g->d_reg.p, g->h_reg.p, (size_t) nreg * sizeof(fa2_region),
+                            hipMemcpyHostToDevice, g->stream), "upload");
+    t1 = fa2_now();
+    g->t_send += t1 - t0;
+    t0 = t1;
+    if (hs.ok()) hs.check(g->use_double ? launch_bh<double>(g, nreg, scaling, theta)
+                                        : launch_bh<float>(g, nreg, scaling, theta), "launch");
+    if (hs.ok()) hs.check(hipStreamSynchronize(g->stream), "compute");
+    t1 = fa2_now();
+    g->t_compute += t1 - t0;
+    t0 = t1;
+    const size_t rs = g->use_double ? sizeof(double) : sizeof(float);
+    if (hs.ok()) hs.check(hipMemcpyAsync(g->h_f.p, g->d_f.p, 2 * n * rs,
+                                         hipMemcpyDeviceToHost, g->stream), "fetch");
+    if (hs.ok()) hs.check(hipStreamSynchronize(g->stream), "fetch");
+    if (!hs.ok()) {
+        set_hip_error("the computation on the GPU failed", hs.err);
+        return 1;
+    }
+    if (g->use_double) {
+        const double *f = (const double *) g->h_f.p;
+        for (size_t k = 0; k < n; k++) {
+            const int i = t->perm[k];
+            dx[i] += f[2 * k];
+            dy[i] += f[2 * k + 1];
+        }
+    } else {
+        const float *f = (const float *) g->h_f.p;
+        for (size_t k = 0; k < n; k++) {
+            const int i = t->perm[k];
+            dx[i] += (double) f[2 * k];
+            dy[i] += (double) f[2 * k + 1];
+        }
+    }
+    g->t_fetch += fa2_now() - t0;
+    return 0;
+}

@@ -1,0 +1,106 @@
+/*
+ * fa2_hip.h — entry points of libfa2hip.so beyond the drop-in ABI of fa2.h.
+ *
+ *  - fa2_layout_run / fa2_layout_large_run: the two .Call entries of the
+ *    reference (C_forceatlas2, src/forceatlas2.c:521-827, and
+ *    C_forceatlas2_large, :835-947) as plain C functions over plain arrays,
+ *    with the same preprocessing (mass = 1 + degree, incidence lists,
+ *    attraction coefficients). They are the host drivers of the two GPU
+ *    paths: the repulsion offload (forceatlas2_gpu) and the all-GPU layout
+ *    (forceatlas2_large).
+ *  - fa2_batch_*: many graphs of uneven size laid out together on one
+ *    device, every iteration of every graph in the same kernel launches,
+ *    with state resident between calls. No counterpart in the reference.
+ *  - fa2_serve_*: a layout server over the batch engine (continuous
+ *    batching, admission by device-memory footprint). No counterpart in the
+ *    reference.
+ */
+#ifndef FA2_HIP_H
+#define FA2_HIP_H
+
+#include <stdint.h>
+
+#include "fa2.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* C_forceatlas2 (src/forceatlas2.c:521-827) without R: n nodes, m edges
+ * from[e] -- to[e] (0-based) of weight w[e]; pos = x then y, updated in
+ * place; size (n radii), fixed (n flags, may be NULL); dpar = (scaling,
+ * gravity, jitter_tolerance, theta); flags = (strong, linlog, outbound,
+ * adjust, barnes_hut); gpu = (device, use_double), device -1 computing the
+ * repulsion on the host. times (7, may be NULL) = total, tree, repulsion,
+ * gpu_send, gpu_compute, gpu_fetch, other. 0 on success, 1 on error. */
+int fa2_layout_run(int n, int m, const int *from, const int *to, const double *w,
+                   double *pos, const double *size, const int *fixed, int iterations,
+                   const double *dpar, const int *flags, int threads, const int *gpu,
+                   double *times);
+
+/* C_forceatlas2_large (src/forceatlas2.c:835-947) without R: flags =
+ * (strong, linlog, outbound, adjust); gpu = (device, use_double, profile).
+ * times (FA2_FULL_NTIMES, may be NULL). 0 on success, 1 on error. */
+int fa2_layout_large_run(int n, int m, const int *from, const int *to, const double *w,
+                         double *pos, const double *size, const int *fixed, int iterations,
+                         const double *dpar, const int *flags, const int *gpu, double *times);
+
+/* The Gephi quadtree of the reference (tree_build, src/forceatlas2.c:64-239)
+ * built on the host, into a tree whose arrays have been allocated by the
+ * caller: reg (2n), perm (n), cnt (8n), buf[0], buf[1] (n), px, py, pm (n),
+ * ps (n, or NULL when size is NULL). */
+void fa2_tree_build(fa2_tree *t, int n, const double *x, const double *y, const double *mass,
+                    const double *size, int threads);
+
+/* ----- batches ---------------------------------------------------------- */
+
+typedef struct fa2_batch fa2_batch;
+
+/* Upload ngraphs graphs (arguments as for fa2_full_run; the device, the
+ * precision and `profile` are those of params[0]) with their initial
+ * positions pos[g] (x then y). params[g].iterations caps the iterations of
+ * graph g: once reached, the graph stops moving. NULL on error. */
+fa2_batch *fa2_batch_open(int ngraphs, const fa2_full_params *params, const double *const *pos);
+
+/* Enqueue `iterations` iterations of every graph; returns without waiting. */
+int fa2_batch_run(fa2_batch *b, int iterations);
+
+/* Wait for the device. 1 on a device error or when a graph produced
+ * invalid numbers. */
+int fa2_batch_sync(fa2_batch *b);
+
+/* Positions (x then y) and last forces (x then y) of graph g. */
+int fa2_batch_positions(fa2_batch *b, int g, double *pos);
+int fa2_batch_forces(fa2_batch *b, int g, double *f);
+
+/* (total swinging, total traction, speed, speed efficiency) of graph g
+ * after its last iteration. */
+int fa2_batch_speed(fa2_batch *b, int g, double out[4]);
+
+/* The Morton tree of graph g at its last iteration, in the numbering of the
+ * reference (regions 0..n-2, leaves n-1..2n-2): sorted keys (n), sorted
+ * permutation (n), children (2(n-1)), ranges (2(n-1)), parents (2n-1). Any
+ * pointer may be NULL. */
+int fa2_batch_tree(fa2_batch *b, int g, uint32_t *keys_sorted, uint32_t *order, int *child,
+                   int *range, int *parent);
+
+/* Device time of each phase (FA2_FULL_NTIMES slots, seconds, slot 0 unused)
+ * summed over the iterations run with profile set. */
+int fa2_batch_stage_times(fa2_batch *b, double *times);
+
+/* Work of the Barnes-Hut repulsion on the tree of the last iteration,
+ * counted by an untimed instrumented traversal: out = (regions accepted as
+ * monopoles, regions opened, node pairs). */
+int fa2_batch_count_work(fa2_batch *b, unsigned long long out[3]);
+
+/* The HIP stream of the batch (hipStream_t), for callers that time kernels
+ * with events on it. */
+void *fa2_batch_stream(fa2_batch *b);
+
+void fa2_batch_close(fa2_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
