@@ -69,7 +69,7 @@ PROTOTYPES = {
     "fa2_gpu_ndevices": (c_int, []),
     "fa2_gpu_device_info": (c_int, [c_int, POINTER(DevInfo)]),
     "fa2_gpu_open": (c_void_p, [c_int, c_int, c_int, c_int, c_int, c_int]),
-    "fa2_gpu_times": (None, [c_void_p, POINTER(c_double)]),
+    "fa2_gpu_times": (None, [c_void_p, c_void_p]),
     "fa2_gpu_close": (None, [c_void_p]),
     "fa2_gpu_repulse_exact": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p,
                                       c_void_p, c_double, c_void_p, c_void_p]),
@@ -101,6 +101,8 @@ PROTOTYPES = {
                                c_void_p, c_void_p]),
     "fa2_batch_stage_times": (c_int, [c_void_p, c_void_p]),
     "fa2_batch_count_work": (c_int, [c_void_p, c_void_p]),
+    "fa2_batch_set_state": (c_int, [c_void_p, c_int, c_void_p, c_void_p,
+                                    c_double, c_double]),
     "fa2_batch_stream": (c_void_p, [c_void_p]),
     "fa2_batch_close": (None, [c_void_p]),
 }

@@ -401,6 +401,17 @@ class Batch:
         return dict(zip(["total", "codes", "sort", "tree", "repulsion",
                          "attraction", "move"], out))
 
+    def set_state(self, g, pos, f_old=None, speed=1.0, efficiency=1.0):
+        """Replace the state of graph g: positions (n, 2), previous force
+        (n, 2) or None, speed and speed efficiency."""
+        n = self.n[g]
+        xy = np.ascontiguousarray(np.asarray(pos, np.float64).T).reshape(-1)
+        fo = None if f_old is None else \
+            np.ascontiguousarray(np.asarray(f_old, np.float64).T).reshape(-1)
+        if lib.fa2_batch_set_state(self.h, g, _ptr(xy), _ptr(fo), float(speed),
+                                   float(efficiency)) != 0:
+            raise GPUError(last_error())
+
     def count_work(self):
         """(accepted regions, opened regions, node pairs) of the repulsion on
         the last tree."""

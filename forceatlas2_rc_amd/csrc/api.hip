@@ -102,6 +102,13 @@ extern "C" int fa2_batch_count_work(fa2_batch *b, unsigned long long out[3])
     return b->e->count_work(out);
 }
 
+extern "C" int fa2_batch_set_state(fa2_batch *b, int g, const double *pos, const double *f_old,
+                                   double speed, double speed_efficiency)
+{
+    if (!b) return 1;
+    return b->e->set_state(g, pos, f_old, speed, speed_efficiency);
+}
+
 extern "C" void *fa2_batch_stream(fa2_batch *b) { return b ? (void *) b->e->stream : nullptr; }
 
 extern "C" void fa2_batch_close(fa2_batch *b) { delete b; }

@@ -39,6 +39,10 @@ struct Engine {
     // Work of the repulsion on the current tree: regions accepted, regions
     // opened, node pairs (untimed instrumented traversal).
     virtual int count_work(unsigned long long out[3]) = 0;
+    // Replace the state of one graph: positions (x then y), previous force
+    // (x then y, NULL for zero), speed and speed efficiency.
+    virtual int set_state(int graph, const double *pos, const double *fold, double speed,
+                          double eff) = 0;
 };
 
 Engine *make_engine(int use_double);

@@ -67,7 +67,7 @@ struct Tile {
     int g, lo, hi, j; // graph, node range [lo, hi), tile index in the graph
 };
 
-template <typename R> struct TNode {
+template <typename R> struct alignas(32) TNode {
     typename Vec<R>::v4 c; // centre x, centre y, mass, size^2
     int4 l;                // left child, skip, first, last (sorted positions)
 };
@@ -377,7 +377,7 @@ __global__ void k_karras(int NI, const int *__restrict__ gi, const GraphDesc *__
                          const uint32_t *__restrict__ ks, int2 *__restrict__ child,
                          TNode<R> *__restrict__ tn, int *__restrict__ parent_leaf,
                          int *__restrict__ parent_int, int *__restrict__ escape,
-                         int *__restrict__ visits, int *__restrict__ ready_count)
+                         int *__restrict__ visits, int NS, int *__restrict__ rcnt)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= NI) return;
@@ -413,7 +413,7 @@ __global__ void k_karras(int NI, const int *__restrict__ gi, const GraphDesc *__
     escape[S - g] = right;
     tn[t].l = make_int4(left, 0, base + first, base + last);
     visits[t] = 0;
-    if (t == 0) *ready_count = 0;
+    for (int q = t; q <= NS; q += NI) rcnt[q] = 0;
 }
 
 template <typename R> struct Summ {
@@ -471,25 +471,51 @@ __device__ inline void merge_region(int t, int g, int gend,
     tn[t].l.y = last == gend - 1 ? kEnd : escape[last - g];
 }
 
-// Leaf to root, first inside each tile: the regions whose nodes all lie in
-// one tile are merged by that tile's workgroup, the second child to arrive
-// merging (arrival counters in LDS, workgroup-scope acquire/release). A
-// region spanning several tiles only counts the arrival in HBM; when both
-// of its children are complete it joins the ready list of k_aggregate_top.
-// No agent-scope fence is needed: the kernel boundary publishes the
-// summaries to the next kernel.
+// Leaf to root in three levels, the second child to arrive at a region
+// merging it:
+//  1. each tile (2048 sorted nodes) by one workgroup, arrival counters in
+//     LDS with workgroup-scope acquire/release;
+//  2. the regions spanning several tiles of one super-tile (64 tiles), by
+//     one workgroup per super-tile;
+//  3. the few regions spanning several super-tiles, by one workgroup.
+// A region spanning a wider block than the current level only counts the
+// arrival in HBM; when both of its children are complete it joins the ready
+// list of the next level. No agent-scope fence is needed: each kernel
+// boundary publishes the summaries to the next level. The super-tile of
+// sorted position p of graph g is sbase_g + (p - noff_g) / kSuper.
+
+constexpr int kSuper = 64 * kTile;
+
+struct Super {
+    int g, lo, hi, pad; // graph, sorted positions [lo, hi)
+};
+
+__device__ inline void push_ready(const GraphDesc &D, int t, int first, int last, int NS, int N,
+                                  int *ready, int *rcnt)
+{
+    const int s0 = D.pad + (first - D.noff) / kSuper, s1 = D.pad + (last - D.noff) / kSuper;
+    if (s0 == s1) {
+        const int q = atomicAdd(&rcnt[s0], 1);
+        ready[D.noff + (s0 - D.pad) * kSuper + q] = t;
+    } else {
+        const int q = atomicAdd(&rcnt[NS], 1);
+        ready[N + q] = t;
+    }
+}
+
 template <typename R>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
                  const typename Vec<R>::v4 *__restrict__ psort, const int2 *__restrict__ child,
                  const int *__restrict__ parent_leaf, const int *__restrict__ parent_int,
-                 const int *__restrict__ escape, int *gvis, int *ready, int *ready_count,
+                 const int *__restrict__ escape, int NS, int N, int *gvis, int *ready, int *rcnt,
                  typename Vec<R>::v4 *nd, TNode<R> *tn)
 {
     __shared__ int lvis[kTile];
     const Tile T = tiles[blockIdx.x];
     const int g = T.g, lo = T.lo, hi = T.hi;
-    const int gend = gd[g].noff + gd[g].n;
+    const GraphDesc D = gd[g];
+    const int gend = D.noff + D.n;
     const int tbase = lo - g; // regions inside the tile are numbered tbase..
     for (int i = threadIdx.x; i < kTile; i += blockDim.x) lvis[i] = 0;
     __syncthreads();
@@ -500,7 +526,7 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
         for (int depth = 0; t >= 0 && depth < 128; depth++) {
             const int4 l = tn[t].l;
             if (l.z < lo || l.w >= hi) { // the region spans several tiles
-                if (atomicAdd(&gvis[t], 1) == 1) ready[atomicAdd(ready_count, 1)] = t;
+                if (atomicAdd(&gvis[t], 1) == 1) push_ready(D, t, l.z, l.w, NS, N, ready, rcnt);
                 break;
             }
             const int old = __hip_atomic_fetch_add(&lvis[t - tbase], 1, __ATOMIC_ACQ_REL,
@@ -512,19 +538,48 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     }
 }
 
-// The regions spanning several tiles (about tiles x depth of them), by one
-// workgroup: from the ready regions up, the second child to arrive merging.
 template <typename R>
 __global__ void __launch_bounds__(1024)
-k_aggregate_top(const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
-                const typename Vec<R>::v4 *__restrict__ psort, const int2 *__restrict__ child,
-                const int *__restrict__ parent_int, const int *__restrict__ escape,
-                const int *__restrict__ ready, const int *__restrict__ ready_count,
-                const int *__restrict__ gi, int *gvis, typename Vec<R>::v4 *nd, TNode<R> *tn)
+k_aggregate_super(const Super *__restrict__ supers, const GraphDesc *__restrict__ gd,
+                  const typename Vec<R>::v4 *__restrict__ psort, const int2 *__restrict__ child,
+                  const int *__restrict__ parent_int, const int *__restrict__ escape, int NS,
+                  int N, int *gvis, int *ready, int *rcnt, typename Vec<R>::v4 *nd, TNode<R> *tn)
 {
-    const int cnt = *ready_count;
+    const Super S = supers[blockIdx.x];
+    const int g = S.g;
+    const GraphDesc D = gd[g];
+    const int gend = D.noff + D.n;
+    const int cnt = rcnt[blockIdx.x];
     for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-        int t = ready[q];
+        int t = ready[S.lo + q];
+        for (int depth = 0; depth < 128; depth++) {
+            merge_region<R>(t, g, gend, psort, child, escape, nd, tn);
+            const int p = parent_int[t];
+            if (p < 0) break;
+            const int4 l = tn[p].l;
+            if (l.z < S.lo || l.w >= S.hi) { // spans several super-tiles
+                if (atomicAdd(&gvis[p], 1) == 1) push_ready(D, p, l.z, l.w, NS, N, ready, rcnt);
+                break;
+            }
+            const int old = __hip_atomic_fetch_add(&gvis[p], 1, __ATOMIC_ACQ_REL,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0) break;
+            t = p;
+        }
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(1024)
+k_aggregate_top(const GraphDesc *__restrict__ gd, const typename Vec<R>::v4 *__restrict__ psort,
+                const int2 *__restrict__ child, const int *__restrict__ parent_int,
+                const int *__restrict__ escape, const int *__restrict__ gi, int NS, int N,
+                const int *__restrict__ ready, const int *__restrict__ rcnt, int *gvis,
+                typename Vec<R>::v4 *nd, TNode<R> *tn)
+{
+    const int cnt = rcnt[NS];
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+        int t = ready[N + q];
         const int g = gi[t];
         const int gend = gd[g].noff + gd[g].n;
         for (int depth = 0; depth < 128; depth++) {
@@ -540,12 +595,58 @@ k_aggregate_top(const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
 }
 
 // ---------------------------------------------------------------------------
-// Repulsion: stackless Barnes-Hut over the Karras tree (pipeline.cl:332-383).
-// One lane per node in sorted (Morton) order, so the lanes of a wave are
-// neighbours in space and walk mostly the same regions. Regions are visited
-// in the reference's order (left child first), so each node accumulates the
-// same terms in the same order as the reference.
+// Repulsion: Barnes-Hut over the Karras tree (pipeline.cl:332-383), walked
+// by a whole wave at once.
+//
+// The 64 lanes of a wave hold 64 consecutive nodes in Morton order, which
+// are neighbours in space and visit mostly the same regions. The wave walks
+// the union of their visits in depth-first order (left child first, along
+// the skip links), one region per step: the region is the same for every
+// lane, so its record is a scalar load and every branch is uniform. A lane
+// takes part in a step only when its own traversal would visit that region:
+// it keeps `done`, the last sorted position covered by the last region it
+// accepted (or visited node by node), and a region is skipped by the lane
+// when it starts at or before `done`. The wave opens a region when at least
+// one taking-part lane does not accept it. Every lane therefore sees
+// exactly the regions, in exactly the order, of the reference traversal,
+// and accumulates the same terms in the same order.
 // ---------------------------------------------------------------------------
+
+// Per-lane force of the traversal: each term is computed in the working
+// precision, the running sum is kept in FP64 (two adds and two conversions
+// per term), so long sums with cancellation keep the precision of the terms.
+template <typename R> struct Sum2 {
+    double x = 0.0, y = 0.0;
+    __device__ inline void add(R tx, R ty)
+    {
+        x += (double) tx;
+        y += (double) ty;
+    }
+};
+
+// The force of node o on this lane's node, added to sum when `on`. With
+// node sizes (prevent_overlap) the term is computed in FP64 from the FP32
+// inputs: the force jumps by orders of magnitude as two discs touch, and the
+// terms of overlapping discs are large and cancel.
+template <typename R, bool ADJUST>
+__device__ inline void lane_pair(typename Vec<R>::v4 me, typename Vec<R>::v4 o, R smi,
+                                 double sscale, bool on, Sum2<R> &sum)
+{
+    const R xd = me.x - o.x, yd = me.y - o.y;
+    if (ADJUST) {
+        const double dx = (double) xd, dy = (double) yd;
+        const double d = __builtin_sqrt(dx * dx + dy * dy) - ((double) me.w + (double) o.w);
+        const double smm = sscale * (double) o.z;
+        double f = d > 0.0 ? smm / (d * d) : (d < 0.0 ? 100.0 * smm : 0.0);
+        f = on ? f : 0.0;
+        sum.x += dx * f;
+        sum.y += dy * f;
+    } else {
+        R f = pair_factor<R>(xd * xd + yd * yd, smi * o.z);
+        f = on ? f : (R) 0;
+        sum.add(xd * f, yd * f);
+    }
+}
 
 // With COUNT, the kernel also counts its work for the roofline: regions
 // accepted (A) and opened (O), and node pairs (P), into cnt[0..2].
@@ -554,64 +655,106 @@ __global__ void __launch_bounds__(256)
 k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
           const typename Vec<R>::v4 *__restrict__ psort, const TNode<R> *__restrict__ tn,
           const int *__restrict__ escape, const uint32_t *__restrict__ order,
-          typename Vec<R>::v2 *__restrict__ F, unsigned long long *cnt = nullptr)
+          double2 *__restrict__ F, unsigned long long *cnt = nullptr)
 {
-    unsigned long long nA = 0, nO = 0, nP = 0;
     using V4 = typename Vec<R>::v4;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
-    const int g = gid[k];
-    const GraphDesc G = gd[g];
-    const int gend = G.noff + G.n;
-    const bool adjust = ANY_ADJUST && (G.flags & GF_ADJUST);
-    const R theta2 = (R) G.theta2;
+    unsigned long long nA = 0, nO = 0, nP = 0;
+    const int k0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = k0 < N;
+    const int k = valid ? k0 : N - 1;
+    const int my_g = valid ? gid[k] : -1;
     const V4 me = psort[k];
-    const R smi = (R) G.scaling * me.z;
-    R ax = (R) 0, ay = (R) 0;
-    int cur = G.n >= 2 ? G.ioff : ~k;
-    // every node of the tree is visited at most once (2n - 1 of them); the
-    // bound only guards against a corrupt tree
-    for (int steps = 2 * G.n; cur != kEnd && steps > 0; steps--) {
-        int j0, j1, nxt;
-        if (cur >= 0) {
-            const V4 c = tn[cur].c;
-            const int4 l = tn[cur].l;
-            const R xd = me.x - c.x, yd = me.y - c.y;
-            const R d2 = xd * xd + yd * yd;
-            if (d2 * theta2 > c.w) {
-                const R f = (smi * c.z) * frcp_r(d2);
-                ax += xd * f;
-                ay += yd * f;
-                cur = l.y;
-                if (COUNT) nA++;
-                continue;
+    Sum2<R> acc2;
+    // the lanes of one wave may belong to several graphs of a batch: walk
+    // each graph's tree in turn with its own lanes
+    uint64_t todo = __ballot(valid);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long) todo) - 1;
+        const int g = __builtin_amdgcn_readlane(my_g, leader);
+        const bool mine = my_g == g;
+        todo &= ~__ballot(mine);
+        const GraphDesc G = gd[g];
+        const int gend = G.noff + G.n;
+        const bool adjust = ANY_ADJUST && (G.flags & GF_ADJUST);
+        const R theta2 = (R) G.theta2;
+        const R smi = (R) G.scaling * me.z;
+        const double sscale = G.scaling * (double) me.z; // scaling * m_i in FP64
+        int done = mine ? INT_MIN + 1 : INT_MAX; // lanes of other graphs never take part
+        int c = G.ioff;                          // the root (every graph has n >= 2)
+        // every node of the tree is visited at most once (2n - 1 of them); the
+        // bound only guards against a corrupt tree
+        for (int steps = 2 * G.n; c != kEnd && steps > 0; steps--) {
+            c = __builtin_amdgcn_readfirstlane(c);
+            int j0, j1, nxt;
+            bool take;
+            if (c >= 0) {
+                const TNode<R> node = tn[c]; // one scalar load of the whole record
+                const V4 rc = node.c;
+                const int4 rl = node.l;
+                if (sizeof(R) == 4) // keep the eight dwords together in SGPRs
+                    asm volatile("" ::"s"(rc.x), "s"(rc.y), "s"(rc.z), "s"(rc.w), "s"(rl.x),
+                                 "s"(rl.y), "s"(rl.z), "s"(rl.w));
+                const R xd = me.x - rc.x, yd = me.y - rc.y;
+                const R d2 = xd * xd + yd * yd;
+                const R smm = smi * rc.z;
+                const bool active = rl.z > done;
+                const bool far = d2 * theta2 > rc.w;
+                const bool acc = active & far;
+                take = active & !far;
+                if (COUNT) {
+                    nA += acc;
+                    nO += take;
+                }
+                if (__ballot(acc)) { // the monopole, for the lanes that accept
+                    const R f = acc ? smm * frcp_r(d2) : (R) 0;
+                    acc2.add(xd * f, yd * f);
+                    done = acc ? rl.w : done;
+                }
+                if (__ballot(take) == 0) { // every lane is done with this region
+                    c = rl.y;
+                    continue;
+                }
+                if (rl.w - rl.z >= kLeafSize) { // descend
+                    c = rl.x;
+                    continue;
+                }
+                j0 = rl.z; // a small region: its nodes one by one
+                j1 = rl.w;
+                nxt = rl.y;
+            } else {
+                j0 = j1 = ~c;
+                take = j0 > done;
+                nxt = j0 == gend - 1 ? kEnd : escape[j0 - g];
             }
-            if (COUNT) nO++;
-            if (l.w - l.z >= kLeafSize) {
-                cur = l.x;
-                continue;
+            for (int j = j0; j <= j1; j++) {
+                const V4 o = psort[j];
+                const bool pair = take & (j != k);
+                if (adjust)
+                    lane_pair<R, true>(me, o, smi, sscale, pair, acc2);
+                else
+                    lane_pair<R, false>(me, o, smi, sscale, pair, acc2);
+                if (COUNT) nP += pair;
             }
-            j0 = l.z;
-            j1 = l.w;
-            nxt = l.y;
-        } else {
-            j0 = j1 = ~cur;
-            nxt = j0 == gend - 1 ? kEnd : escape[j0 - g];
+            done = take ? j1 : done;
+            c = nxt;
         }
-        for (int j = j0; j <= j1; j++) {
-            if (j == k) continue;
-            const V4 o = psort[j];
-            const R xd = me.x - o.x, yd = me.y - o.y;
-            const R d2 = xd * xd + yd * yd;
-            const R f = adjust ? pair_factor<R, true>(d2, smi * o.z, me.w + o.w)
-                               : pair_factor<R, false>(d2, smi * o.z, (R) 0);
-            ax += xd * f;
-            ay += yd * f;
-            if (COUNT) nP++;
-        }
-        cur = nxt;
     }
-    F[order[k]] = mk2<R>(ax, ay);
+    if (valid) {
+        // gravity (pipeline.cl:404-411): the force leaves this kernel as
+        // repulsion minus gravity, the attraction kernels add the edges
+        double fx = acc2.x, fy = acc2.y;
+        const GraphDesc G = gd[my_g];
+        const R grav = (R) G.gravity;
+        if (grav != (R) 0) {
+            const R dist = fsqrt_r(me.x * me.x + me.y * me.y);
+            if (dist > (R) 0) {
+                const R gg = (G.flags & GF_STRONG) ? grav * me.z : grav * me.z / dist;
+                fx -= (double) (me.x * gg);
+                fy -= (double) (me.y * gg);
+            }
+        }
+        F[order[k]] = make_double2(fx, fy);
+    }
     if (COUNT) {
         atomicAdd(&cnt[0], nA);
         atomicAdd(&cnt[1], nO);
@@ -620,88 +763,139 @@ k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
 }
 
 // ---------------------------------------------------------------------------
-// Gravity and attraction (pipeline.cl:392-429), degree-binned: a node of the
-// class GL is handled by GL lanes that stride over its incidences; the lanes
-// are combined with a fixed butterfly.
+// Attraction along the edges (pipeline.cl:392-429), added to the force that
+// the repulsion kernel left (repulsion minus gravity). Balanced by degree:
+//  * nodes of 1..16 incidences: one lane each;
+//  * 17..256: 16 lanes each, combined with a fixed butterfly;
+//  * more: work items of at most kChunk consecutive incidences, one wave
+//    each; a node cut into several items gets the sum of their partial sums
+//    in item order (k_attract_fix). Nodes without edges are not visited.
 // ---------------------------------------------------------------------------
+
+constexpr int kChunk = 1024;
+
+template <typename R>
+__device__ inline void attract_terms(int e0, int e1, int step, bool linlog, bool adjust,
+                                     typename Vec<R>::v2 pi, R si,
+                                     const typename Vec<R>::v2 *__restrict__ pos,
+                                     const R *__restrict__ size, const int *__restrict__ inc_node,
+                                     const R *__restrict__ inc_coef, KahanSum2<R> &sum)
+{
+    for (int e = e0; e < e1; e += step) {
+        const int j = inc_node[e];
+        const typename Vec<R>::v2 pj = pos[j];
+        const R xd = pi.x - pj.x, yd = pi.y - pj.y;
+        R f = -inc_coef[e];
+        if (adjust) {
+            // the gap between the discs, in FP64 as for the repulsion: the
+            // edge only pulls while it is positive
+            const double dx = (double) xd, dy = (double) yd;
+            const double gap = __builtin_sqrt(dx * dx + dy * dy) - ((double) si + (double) size[j]);
+            if (!(gap > 0.0)) continue;
+            if (linlog) {
+                const R dist = (R) gap;
+                f *= flog1p_r(dist) / dist;
+            }
+        } else if (linlog) {
+            const R dist = fsqrt_r(xd * xd + yd * yd);
+            if (!(dist > (R) 0)) continue;
+            f *= flog1p_r(dist) / dist;
+        }
+        sum.add(xd * f, yd * f);
+    }
+}
 
 template <typename R, int GL>
 __global__ void __launch_bounds__(256)
 k_attract(int count, const int *__restrict__ nodes, const int *__restrict__ gid,
           const GraphDesc *__restrict__ gd, const typename Vec<R>::v2 *__restrict__ pos,
-          const R *__restrict__ mass, const R *__restrict__ size, const int *__restrict__ inc_off,
+          const R *__restrict__ size, const int *__restrict__ inc_off,
           const int *__restrict__ inc_node, const R *__restrict__ inc_coef,
-          typename Vec<R>::v2 *__restrict__ F)
+          double2 *__restrict__ F)
 {
-    __shared__ R red[2][4];
-    int slot, sub;
-    if (GL <= 64) {
-        const int gpb = 256 / GL; // groups per block
-        slot = blockIdx.x * gpb + (int) threadIdx.x / GL;
-        sub = (int) threadIdx.x % GL;
-    } else {
-        slot = blockIdx.x;
-        sub = threadIdx.x;
-    }
+    const int slot = blockIdx.x * (256 / GL) + (int) threadIdx.x / GL;
+    const int sub = (int) threadIdx.x % GL;
     const bool live = slot < count;
-    const int i = live ? nodes[slot] : 0;
-    const int g = gid[i];
-    const GraphDesc G = gd[g];
-    const bool linlog = G.flags & GF_LINLOG, adjust = G.flags & GF_ADJUST;
-    const typename Vec<R>::v2 pi = pos[i];
-    const R si = size[i];
-    R ax = (R) 0, ay = (R) 0;
-    if (live) {
-        const int e1 = inc_off[i + 1];
-        for (int e = inc_off[i] + sub; e < e1; e += GL) {
-            const int j = inc_node[e];
-            const typename Vec<R>::v2 pj = pos[j];
-            const R xd = pi.x - pj.x, yd = pi.y - pj.y;
-            R f = -inc_coef[e];
-            if (adjust || linlog) {
-                R dist = fsqrt_r(xd * xd + yd * yd);
-                if (adjust) dist -= si + size[j];
-                if (!(dist > (R) 0)) continue;
-                if (linlog) f *= flog1p_r(dist) / dist;
-            }
-            ax += xd * f;
-            ay += yd * f;
-        }
-    }
-    if (GL > 1) {
-        const int width = GL < 64 ? GL : 64;
+    const int i = live ? nodes[slot] : nodes[0];
+    const int flags = gd[gid[i]].flags;
+    KahanSum2<R> sum;
+    if (live)
+        attract_terms<R>(inc_off[i] + sub, inc_off[i + 1], GL, flags & GF_LINLOG,
+                         flags & GF_ADJUST, pos[i], size[i], pos, size, inc_node, inc_coef, sum);
+    R ax = sum.sum_x(), ay = sum.sum_y();
 #pragma unroll
-        for (int o = width / 2; o >= 1; o >>= 1) {
-            ax += __shfl_xor(ax, o, 64);
-            ay += __shfl_xor(ay, o, 64);
-        }
-        if (GL > 64) {
-            const int w = threadIdx.x >> 6;
-            if (lane_id() == 0) {
-                red[0][w] = ax;
-                red[1][w] = ay;
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                ax = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
-                ay = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
-            }
-        }
+    for (int o = GL / 2; o >= 1; o >>= 1) {
+        ax += __shfl_xor(ax, o, 64);
+        ay += __shfl_xor(ay, o, 64);
     }
     if (!live || sub != 0) return;
-    typename Vec<R>::v2 f = F[i];
-    const R grav = (R) G.gravity;
-    if (grav != (R) 0) {
-        const R dist = fsqrt_r(pi.x * pi.x + pi.y * pi.y);
-        if (dist > (R) 0) {
-            const R gg = (G.flags & GF_STRONG) ? grav * mass[i] : grav * mass[i] / dist;
-            f.x -= pi.x * gg;
-            f.y -= pi.y * gg;
-        }
-    }
-    f.x += ax;
-    f.y += ay;
+    double2 f = F[i];
+    f.x += (double) ax;
+    f.y += (double) ay;
     F[i] = f;
+}
+
+// One wave per item (node, first incidence, end, partial slot or -1).
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict__ gid,
+                const GraphDesc *__restrict__ gd, const typename Vec<R>::v2 *__restrict__ pos,
+                const R *__restrict__ size, const int *__restrict__ inc_node,
+                const R *__restrict__ inc_coef, double2 *__restrict__ F,
+                typename Vec<R>::v2 *__restrict__ part)
+{
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= count) return;
+    const int lane = lane_id();
+    const int4 it = items[w];
+    const int i = it.x;
+    const int flags = gd[gid[i]].flags;
+    KahanSum2<R> sum;
+    attract_terms<R>(it.y + lane, it.z, 64, flags & GF_LINLOG, flags & GF_ADJUST, pos[i], size[i],
+                     pos, size, inc_node, inc_coef, sum);
+    R ax = sum.sum_x(), ay = sum.sum_y();
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        ax += __shfl_xor(ax, o, 64);
+        ay += __shfl_xor(ay, o, 64);
+    }
+    if (lane != 0) return;
+    if (it.w < 0) {
+        double2 f = F[i];
+        f.x += (double) ax;
+        f.y += (double) ay;
+        F[i] = f;
+    } else {
+        part[it.w] = mk2<R>(ax, ay);
+    }
+}
+
+// One wave per node cut into several items: (node, first slot, slots).
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_attract_fix(int count, const int4 *__restrict__ hubs, const typename Vec<R>::v2 *__restrict__ part,
+              double2 *__restrict__ F)
+{
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= count) return;
+    const int lane = lane_id();
+    const int4 h = hubs[w];
+    KahanSum2<R> sum;
+    for (int q = lane; q < h.z; q += 64) {
+        const typename Vec<R>::v2 v = part[h.y + q];
+        sum.add(v.x, v.y);
+    }
+    R ax = sum.sum_x(), ay = sum.sum_y();
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        ax += __shfl_xor(ax, o, 64);
+        ay += __shfl_xor(ay, o, 64);
+    }
+    if (lane != 0) return;
+    double2 f = F[h.x];
+    f.x += (double) ax;
+    f.y += (double) ay;
+    F[h.x] = f;
 }
 
 // ---------------------------------------------------------------------------
@@ -711,12 +905,10 @@ k_attract(int count, const int *__restrict__ nodes, const int *__restrict__ gid,
 // fixed order.
 // ---------------------------------------------------------------------------
 
-template <typename R>
-__device__ inline void swing_node(typename Vec<R>::v2 f, typename Vec<R>::v2 fo, double m,
-                                  double &sw, double &tr)
+__device__ inline void swing_node(double2 f, double2 fo, double m, double &sw, double &tr)
 {
-    const double sx = (double) fo.x - (double) f.x, sy = (double) fo.y - (double) f.y;
-    const double tx = (double) fo.x + (double) f.x, ty = (double) fo.y + (double) f.y;
+    const double sx = fo.x - f.x, sy = fo.y - f.y;
+    const double tx = fo.x + f.x, ty = fo.y + f.y;
     sw = m * __builtin_sqrt(sx * sx + sy * sy);
     tr = 0.5 * m * __builtin_sqrt(tx * tx + ty * ty);
 }
@@ -725,7 +917,7 @@ template <typename R>
 __global__ void __launch_bounds__(256)
 k_swing(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
         const R *__restrict__ mass, const uint8_t *__restrict__ fixed,
-        const typename Vec<R>::v2 *__restrict__ F, const typename Vec<R>::v2 *__restrict__ Fold,
+        const double2 *__restrict__ F, const double2 *__restrict__ Fold,
         double2 *__restrict__ part)
 {
     __shared__ double red[2][4];
@@ -734,7 +926,7 @@ k_swing(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
     double a = 0.0, b = 0.0;
     for (int i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
         double sw, tr;
-        swing_node<R>(F[i], Fold[i], (double) mass[i], sw, tr);
+        swing_node(F[i], Fold[i], (double) mass[i], sw, tr);
         if (use_fixed && fixed[i]) continue;
         a += sw;
         b += tr;
@@ -825,8 +1017,8 @@ template <typename R>
 __global__ void __launch_bounds__(256)
 k_apply(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
         const GraphState *__restrict__ gs, const R *__restrict__ mass,
-        const uint8_t *__restrict__ fixed, const typename Vec<R>::v2 *__restrict__ F,
-        const typename Vec<R>::v2 *__restrict__ Fold, typename Vec<R>::v2 *__restrict__ pos,
+        const uint8_t *__restrict__ fixed, const double2 *__restrict__ F,
+        const double2 *__restrict__ Fold, typename Vec<R>::v2 *__restrict__ pos,
         double2 *__restrict__ acc, typename Vec<R>::v4 *__restrict__ tbox)
 {
     __shared__ R lds[16];
@@ -840,9 +1032,9 @@ k_apply(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
     for (int i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
         typename Vec<R>::v2 p = pos[i];
         if (move && !(use_fixed && fixed[i])) {
-            const typename Vec<R>::v2 f = F[i];
+            const double2 f = F[i];
             double sw, tr;
-            swing_node<R>(f, Fold[i], (double) mass[i], sw, tr);
+            swing_node(f, Fold[i], (double) mass[i], sw, tr);
             double factor;
             if (adjust) {
                 const double fx = (double) f.x, fy = (double) f.y;
@@ -889,7 +1081,9 @@ template <typename R> struct EngineImpl final : Engine {
     long long I = 0;
     std::vector<GraphDesc> h_gd;
     std::vector<Tile> h_tiles;
-    std::vector<int> h_class[4];
+    std::vector<int> h_class[2];
+    std::vector<int4> h_items, h_hubs;
+    int nslots = 0;
     bool any_adjust = false;
     int parity = 0; // F[parity] receives the force of the next iteration
 
@@ -897,18 +1091,25 @@ template <typename R> struct EngineImpl final : Engine {
     DevBuf<GraphState> d_gs;
     DevBuf<Tile> d_tiles;
     DevBuf<int> d_gid, d_gi;
-    DevBuf<V2> d_pos, d_F[2];
+    DevBuf<V2> d_pos;
+    DevBuf<double2> d_F[2]; // forces are kept in FP64: the swing is their difference
     DevBuf<double2> d_acc;
     DevBuf<R> d_mass, d_size, d_coef;
     DevBuf<uint8_t> d_fixed;
     DevBuf<int> d_inc_off, d_inc_node;
-    DevBuf<int> d_class[4];
+    DevBuf<int> d_class[2];
+    DevBuf<int4> d_items, d_hubs;
+    DevBuf<V2> d_hubpart;
     DevBuf<V4> d_tbox, d_gbox, d_psort, d_nd;
     DevBuf<uint32_t> d_keyA, d_valA, d_keyB, d_valB, d_hist, d_rowtot;
     DevBuf<int2> d_child;
     DevBuf<TNode<R>> d_tn;
     DevBuf<int> d_parent_leaf, d_parent_int, d_escape, d_visits;
-    DevBuf<int> d_ready; // regions spanning tiles whose children are done, then their count
+    DevBuf<int> d_ready; // ready lists: per super-tile at its first position, then the top one
+    DevBuf<int> d_rcnt;  // their lengths (super-tiles, then the top list)
+    DevBuf<Super> d_supers;
+    std::vector<Super> h_supers;
+    int NS = 0;
     DevBuf<double2> d_part;
 
     // profiling: events between the phases of each iteration
@@ -933,6 +1134,8 @@ template <typename R> struct EngineImpl final : Engine {
     int stage_times(double *t) override;
     int graph_status(int graph) override;
     int count_work(unsigned long long out[3]) override;
+    int set_state(int graph, const double *pos, const double *fold, double speed,
+                  double eff) override;
 
     void enqueue_iteration(std::vector<hipEvent_t> *ev);
 };
@@ -981,6 +1184,15 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         D.theta2 = p->theta * p->theta;
         D.jt = p->jitter_tolerance;
         any_adjust = any_adjust || p->adjust;
+        D.pad = (int) h_supers.size(); // first super-tile of the graph
+        for (int s0 = 0; s0 < p->n; s0 += kSuper) {
+            Super S;
+            S.g = g;
+            S.lo = noff + s0;
+            S.hi = std::min(noff + p->n, S.lo + kSuper);
+            S.pad = 0;
+            h_supers.push_back(S);
+        }
         for (int j = 0; j < D.ntiles; j++) {
             Tile t;
             t.g = g;
@@ -993,6 +1205,7 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         toff += D.ntiles;
     }
     T = toff;
+    NS = (int) h_supers.size();
 
     // packed host arrays
     const size_t nn = (size_t) N;
@@ -1031,11 +1244,23 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         ioff += ni;
     }
     inc_off[nn] = (int) ioff;
-    // degree classes of the attraction: 1, 16, 64 or 256 lanes per node
+    // attraction work: one lane per node up to 16 incidences, 16 lanes up
+    // to 256, then waves over chunks of kChunk incidences
     for (int i = 0; i < N; i++) {
-        const int deg = inc_off[i + 1] - inc_off[i];
-        const int c = deg <= 16 ? 0 : deg <= 256 ? 1 : deg <= 4096 ? 2 : 3;
-        h_class[c].push_back(i);
+        const int e0 = inc_off[i], e1 = inc_off[i + 1], deg = e1 - e0;
+        if (deg == 0) continue;
+        if (deg <= 16) {
+            h_class[0].push_back(i);
+        } else if (deg <= 256) {
+            h_class[1].push_back(i);
+        } else if (deg <= kChunk) {
+            h_items.push_back(make_int4(i, e0, e1, -1));
+        } else {
+            const int first = nslots;
+            for (int e = e0; e < e1; e += kChunk)
+                h_items.push_back(make_int4(i, e, std::min(e1, e + kChunk), nslots++));
+            h_hubs.push_back(make_int4(i, first, nslots - first, 0));
+        }
     }
 
     HipStatus hs;
@@ -1057,7 +1282,10 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     hs.check(d_inc_off.alloc(nn + 1), "alloc");
     hs.check(d_inc_node.alloc((size_t) I), "alloc");
     hs.check(d_coef.alloc((size_t) I), "alloc");
-    for (int c = 0; c < 4; c++) hs.check(d_class[c].alloc(h_class[c].size()), "alloc");
+    for (int c = 0; c < 2; c++) hs.check(d_class[c].alloc(h_class[c].size()), "alloc");
+    hs.check(d_items.alloc(h_items.size()), "alloc");
+    hs.check(d_hubs.alloc(h_hubs.size()), "alloc");
+    hs.check(d_hubpart.alloc(nslots), "alloc");
     hs.check(d_tbox.alloc(T), "alloc");
     hs.check(d_gbox.alloc(G), "alloc");
     hs.check(d_psort.alloc(nn), "alloc");
@@ -1074,7 +1302,9 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     hs.check(d_parent_int.alloc(NI), "alloc");
     hs.check(d_escape.alloc(NI), "alloc");
     hs.check(d_visits.alloc(NI), "alloc");
-    hs.check(d_ready.alloc((size_t) NI + 1), "alloc");
+    hs.check(d_ready.alloc((size_t) N + NI), "alloc");
+    hs.check(d_rcnt.alloc((size_t) NS + 1), "alloc");
+    hs.check(d_supers.alloc(NS), "alloc");
     hs.check(d_part.alloc(T), "alloc");
     if (!hs.ok()) {
         set_hip_error("could not allocate memory on the device", hs.err);
@@ -1089,6 +1319,7 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     up(d_gd.p, h_gd.data(), G * sizeof(GraphDesc));
     up(d_gs.p, gs.data(), G * sizeof(GraphState));
     up(d_tiles.p, h_tiles.data(), T * sizeof(Tile));
+    up(d_supers.p, h_supers.data(), NS * sizeof(Super));
     up(d_gid.p, gid.data(), nn * sizeof(int));
     if (NI > 0) up(d_gi.p, gi.data(), (size_t) NI * sizeof(int));
     up(d_pos.p, pos.data(), nn * sizeof(V2));
@@ -1099,9 +1330,11 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     up(d_inc_off.p, inc_off.data(), (nn + 1) * sizeof(int));
     up(d_inc_node.p, inc_node.data(), (size_t) I * sizeof(int));
     up(d_coef.p, coef.data(), (size_t) I * sizeof(R));
-    for (int c = 0; c < 4; c++) up(d_class[c].p, h_class[c].data(), h_class[c].size() * sizeof(int));
-    if (hs.ok()) hs.check(hipMemsetAsync(d_F[0].p, 0, nn * sizeof(V2), stream), "memset");
-    if (hs.ok()) hs.check(hipMemsetAsync(d_F[1].p, 0, nn * sizeof(V2), stream), "memset");
+    for (int c = 0; c < 2; c++) up(d_class[c].p, h_class[c].data(), h_class[c].size() * sizeof(int));
+    up(d_items.p, h_items.data(), h_items.size() * sizeof(int4));
+    up(d_hubs.p, h_hubs.data(), h_hubs.size() * sizeof(int4));
+    if (hs.ok()) hs.check(hipMemsetAsync(d_F[0].p, 0, nn * sizeof(double2), stream), "memset");
+    if (hs.ok()) hs.check(hipMemsetAsync(d_F[1].p, 0, nn * sizeof(double2), stream), "memset");
     if (hs.ok()) {
         k_tile_bbox<R><<<T, 256, 0, stream>>>(d_tiles.p, d_pos.p, d_tbox.p);
         hs.check(hipGetLastError(), "launch");
@@ -1130,7 +1363,7 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
         (void) hipEventRecord(e, stream);
         ev->push_back(e);
     };
-    V2 *Fc = d_F[parity].p, *Fo = d_F[1 - parity].p;
+    double2 *Fc = d_F[parity].p, *Fo = d_F[1 - parity].p;
     mark();
     // 1-2: codes
     k_graph_bbox<R><<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_tbox.p, d_gbox.p);
@@ -1153,16 +1386,22 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
     if (NI > 0) {
         k_karras<R><<<ceil_div(NI, 256), 256, 0, stream>>>(
             NI, d_gi.p, d_gd.p, d_keyA.p, d_child.p, d_tn.p, d_parent_leaf.p, d_parent_int.p,
-            d_escape.p, d_visits.p, d_ready.p + NI);
-        k_aggregate_tile<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_psort.p, d_child.p,
-                                                    d_parent_leaf.p, d_parent_int.p, d_escape.p,
-                                                    d_visits.p, d_ready.p, d_ready.p + NI,
-                                                    d_nd.p, d_tn.p);
-        if (T > G) // some graph has several tiles
-            k_aggregate_top<R><<<1, 1024, 0, stream>>>(d_gid.p, d_gd.p, d_psort.p, d_child.p,
-                                                        d_parent_int.p, d_escape.p, d_ready.p,
-                                                        d_ready.p + NI, d_gi.p, d_visits.p,
-                                                        d_nd.p, d_tn.p);
+            d_escape.p, d_visits.p, NS, d_rcnt.p);
+        k_aggregate_tile<R><<<T, 1024, 0, stream>>>(d_tiles.p, d_gd.p, d_psort.p, d_child.p,
+                                                     d_parent_leaf.p, d_parent_int.p, d_escape.p,
+                                                     NS, N, d_visits.p, d_ready.p, d_rcnt.p,
+                                                     d_nd.p, d_tn.p);
+        if (T > G) { // some graph spans several tiles
+            k_aggregate_super<R><<<NS, 1024, 0, stream>>>(d_supers.p, d_gd.p, d_psort.p,
+                                                          d_child.p, d_parent_int.p, d_escape.p,
+                                                          NS, N, d_visits.p, d_ready.p, d_rcnt.p,
+                                                          d_nd.p, d_tn.p);
+            if (NS > G) // and some graph several super-tiles
+                k_aggregate_top<R><<<1, 1024, 0, stream>>>(d_gd.p, d_psort.p, d_child.p,
+                                                            d_parent_int.p, d_escape.p, d_gi.p, NS,
+                                                            N, d_ready.p, d_rcnt.p, d_visits.p,
+                                                            d_nd.p, d_tn.p);
+        }
     }
     mark();
     // 5: repulsion
@@ -1173,26 +1412,24 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
         k_repulse<R, false><<<ceil_div(N, 256), 256, 0, stream>>>(
             N, d_gid.p, d_gd.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fc);
     mark();
-    // 6: gravity and attraction
+    // 6: attraction (gravity was applied with the repulsion)
     {
         const int c0 = (int) h_class[0].size(), c1 = (int) h_class[1].size();
-        const int c2 = (int) h_class[2].size(), c3 = (int) h_class[3].size();
+        const int ci = (int) h_items.size(), ch = (int) h_hubs.size();
         if (c0)
             k_attract<R, 1><<<ceil_div(c0, 256), 256, 0, stream>>>(
-                c0, d_class[0].p, d_gid.p, d_gd.p, d_pos.p, d_mass.p, d_size.p, d_inc_off.p,
-                d_inc_node.p, d_coef.p, Fc);
+                c0, d_class[0].p, d_gid.p, d_gd.p, d_pos.p, d_size.p, d_inc_off.p, d_inc_node.p,
+                d_coef.p, Fc);
         if (c1)
             k_attract<R, 16><<<ceil_div(c1, 16), 256, 0, stream>>>(
-                c1, d_class[1].p, d_gid.p, d_gd.p, d_pos.p, d_mass.p, d_size.p, d_inc_off.p,
-                d_inc_node.p, d_coef.p, Fc);
-        if (c2)
-            k_attract<R, 64><<<ceil_div(c2, 4), 256, 0, stream>>>(
-                c2, d_class[2].p, d_gid.p, d_gd.p, d_pos.p, d_mass.p, d_size.p, d_inc_off.p,
-                d_inc_node.p, d_coef.p, Fc);
-        if (c3)
-            k_attract<R, 256><<<c3, 256, 0, stream>>>(c3, d_class[3].p, d_gid.p, d_gd.p,
-                                                       d_pos.p, d_mass.p, d_size.p, d_inc_off.p,
-                                                       d_inc_node.p, d_coef.p, Fc);
+                c1, d_class[1].p, d_gid.p, d_gd.p, d_pos.p, d_size.p, d_inc_off.p, d_inc_node.p,
+                d_coef.p, Fc);
+        if (ci)
+            k_attract_items<R><<<ceil_div(ci, 4), 256, 0, stream>>>(
+                ci, d_items.p, d_gid.p, d_gd.p, d_pos.p, d_size.p, d_inc_node.p, d_coef.p, Fc,
+                d_hubpart.p);
+        if (ch)
+            k_attract_fix<R><<<ceil_div(ch, 4), 256, 0, stream>>>(ch, d_hubs.p, d_hubpart.p, Fc);
     }
     mark();
     // 7-8: swinging, speed, displacement
@@ -1319,9 +1556,9 @@ template <typename R> int EngineImpl<R>::forces(int graph, double *out)
     (void) hipSetDevice(device);
     const GraphDesc &D = h_gd[graph];
     const size_t n = (size_t) D.n;
-    std::vector<V2> h(n);
+    std::vector<double2> h(n);
     // the last force is in F[1 - parity]
-    hipError_t e = hipMemcpyAsync(h.data(), d_F[1 - parity].p + D.noff, n * sizeof(V2),
+    hipError_t e = hipMemcpyAsync(h.data(), d_F[1 - parity].p + D.noff, n * sizeof(double2),
                                   hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) {
@@ -1329,8 +1566,8 @@ template <typename R> int EngineImpl<R>::forces(int graph, double *out)
         return 1;
     }
     for (size_t i = 0; i < n; i++) {
-        out[i] = (double) h[i].x;
-        out[n + i] = (double) h[i].y;
+        out[i] = h[i].x;
+        out[n + i] = h[i].y;
     }
     return 0;
 }
@@ -1419,7 +1656,7 @@ template <typename R> int EngineImpl<R>::count_work(unsigned long long out[3])
     if (e == hipSuccess) e = hipMemsetAsync(c.p, 0, 3 * sizeof(unsigned long long), stream);
     // the tree of the last iteration; the forces go to the buffer that the
     // next iteration overwrites first
-    V2 *Fs = d_F[parity].p;
+    double2 *Fs = d_F[parity].p;
     if (e == hipSuccess) {
         if (any_adjust)
             k_repulse<R, true, true><<<ceil_div(N, 256), 256, 0, stream>>>(
@@ -1434,6 +1671,51 @@ template <typename R> int EngineImpl<R>::count_work(unsigned long long out[3])
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) {
         set_hip_error("could not count the work of the repulsion", e);
+        return 1;
+    }
+    return 0;
+}
+
+template <typename R>
+int EngineImpl<R>::set_state(int graph, const double *pos, const double *fold, double speed,
+                             double eff)
+{
+    if (graph < 0 || graph >= G) {
+        set_message("no graph %d in this batch", graph);
+        return 1;
+    }
+    (void) hipSetDevice(device);
+    const GraphDesc &D = h_gd[graph];
+    const size_t n = (size_t) D.n;
+    std::vector<V2> p(n);
+    std::vector<double2> fo(n);
+    std::vector<double2> a(Acc<R>::separate ? n : 0);
+    for (size_t i = 0; i < n; i++) {
+        p[i] = mk2<R>((R) pos[i], (R) pos[n + i]);
+        if (Acc<R>::separate) a[i] = make_double2(pos[i], pos[n + i]);
+        fo[i] = fold ? make_double2(fold[i], fold[n + i]) : make_double2(0.0, 0.0);
+    }
+    GraphState st;
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipMemcpy(&st, d_gs.p + graph, sizeof(st), hipMemcpyDeviceToHost);
+    st.speed = speed;
+    st.eff = eff;
+    st.status = 0;
+    if (e == hipSuccess) e = hipMemcpy(d_gs.p + graph, &st, sizeof(st), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_pos.p + D.noff, p.data(), n * sizeof(V2), hipMemcpyHostToDevice);
+    if (e == hipSuccess && Acc<R>::separate)
+        e = hipMemcpy(d_acc.p + D.noff, a.data(), n * sizeof(double2), hipMemcpyHostToDevice);
+    // the next iteration reads the previous force from F[1 - parity]
+    if (e == hipSuccess)
+        e = hipMemcpy(d_F[1 - parity].p + D.noff, fo.data(), n * sizeof(double2),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        k_tile_bbox<R><<<T, 256, 0, stream>>>(d_tiles.p, d_pos.p, d_tbox.p);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        set_hip_error("could not set the state of the graph", e);
         return 1;
     }
     return 0;

@@ -48,18 +48,43 @@ __device__ inline double fsqrt_r(double x) { return __builtin_sqrt(x); }
 __device__ inline float flog1p_r(float x) { return log1pf(x); }
 __device__ inline double flog1p_r(double x) { return log1p(x); }
 
-// Repulsion factor between two nodes (reference rep_factor,
-// inst/opencl/pipeline.cl:46-55); smm = scaling * m_i * m_j, ss = s_i + s_j.
-template <typename R, bool ADJUST> __device__ inline R pair_factor(R d2, R smm, R ss)
+// Repulsion factor between two nodes without sizes (reference rep_factor,
+// inst/opencl/pipeline.cl:46-55); smm = scaling * m_i * m_j.
+template <typename R> __device__ inline R pair_factor(R d2, R smm)
 {
-    if (ADJUST) {
-        const R d = fsqrt_r(d2) - ss;
-        if (d > (R) 0) return smm * frcp_r(d * d);
-        if (d < (R) 0) return (R) 100 * smm;
-        return (R) 0;
-    }
     return d2 > (R) 0 ? smm * frcp_r(d2) : (R) 0;
 }
+
+// The same with node sizes (prevent_overlap): d = |x_i - x_j| - (s_i + s_j),
+// scaling m_i m_j / d^2 apart, 100 scaling m_i m_j overlapping. The gap d is
+// a difference of nearly equal numbers near contact, where the force grows
+// as 1/d^2: it is computed in FP64 from the FP32 inputs, so that its relative
+// error does not grow as the nodes touch.
+template <typename R> __device__ inline R pair_factor_adjust(R xd, R yd, R smm, R si, R sj)
+{
+    const double dx = (double) xd, dy = (double) yd;
+    const double d = __builtin_sqrt(dx * dx + dy * dy) - ((double) si + (double) sj);
+    if (d > 0.0) return (R) ((double) smm / (d * d));
+    if (d < 0.0) return (R) 100 * smm;
+    return (R) 0;
+}
+
+// Compensated (Kahan) sum of a 2-vector in the working precision: long sums
+// of terms of both signs keep about twice the precision of a plain sum.
+template <typename R> struct KahanSum2 {
+    R x = (R) 0, y = (R) 0, cx = (R) 0, cy = (R) 0;
+    __device__ inline void add(R tx, R ty)
+    {
+        const R ux = tx - cx, uy = ty - cy;
+        const R sx = x + ux, sy = y + uy;
+        cx = (sx - x) - ux;
+        cy = (sy - y) - uy;
+        x = sx;
+        y = sy;
+    }
+    __device__ inline R sum_x() const { return x - cx; }
+    __device__ inline R sum_y() const { return y - cy; }
+};
 
 // Lane index inside the wave64.
 __device__ inline int lane_id() { return (int) __lane_id(); }

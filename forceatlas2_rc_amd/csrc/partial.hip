@@ -10,8 +10,9 @@
 //    and read back as broadcasts. Target blocks x source slices fill the 256
 //    CUs even for small n. The per-slice partial forces are summed in a
 //    fixed order, in FP64, by a second kernel: results are reproducible.
-//  * The bulk arithmetic is FP32 with v_rcp_f32 for the 1/d^2 (1 ulp); in
-//    double-precision sessions it is FP64 with IEEE division.
+//  * The bulk arithmetic is FP32 with v_rcp_f32 for the 1/d^2 (1 ulp).
+//    Double-precision sessions use the kernels of partial_f64.hip, which
+//    repeat the reference's CPU operations in order.
 //  * Barnes-Hut: one lane per node in tree order, stackless traversal along
 //    the skip links of the host tree, regions packed as two 16-byte records.
 #include <cmath>
@@ -21,7 +22,10 @@
 
 #include "common.h"
 #include "kernels_common.h"
+#include "partial_f64.h"
 #include "../../include/fa2.h"
+
+#include <type_traits>
 
 using namespace fa2;
 
@@ -91,14 +95,8 @@ k_exact(int n, int slice_len, R scaling, const typename Vec<R>::v4 *__restrict__
             for (int t = 0; t < kExactTargets; t++) {
                 const R xd = xi[t] - s.x, yd = yi[t] - s.y;
                 const R d2 = xd * xd + yd * yd;
-                R f;
-                if (ADJUST) {
-                    const R d = fsqrt_r(d2) - (si[t] + s.w);
-                    f = d > (R) 0 ? (mi[t] * s.z) * frcp_r(d * d)
-                                  : (d < (R) 0 ? (R) 100 * (mi[t] * s.z) : (R) 0);
-                } else {
-                    f = d2 > (R) 0 ? (mi[t] * s.z) * frcp_r(d2) : (R) 0;
-                }
+                const R f = ADJUST ? pair_factor_adjust<R>(xd, yd, mi[t] * s.z, si[t], s.w)
+                                   : pair_factor<R>(d2, mi[t] * s.z);
                 ax[t] += xd * f;
                 ay[t] += yd * f;
             }
@@ -184,7 +182,8 @@ k_bh_host_tree(int n, R scaling, R theta2, const typename Vec<R>::v4 *__restrict
             const V4 o = p[j];
             const R xd = me.x - o.x, yd = me.y - o.y;
             const R d2 = xd * xd + yd * yd;
-            const R g = pair_factor<R, ADJUST>(d2, smi * o.z, me.w + o.w);
+            const R g = ADJUST ? pair_factor_adjust<R>(xd, yd, smi * o.z, me.w, o.w)
+                               : pair_factor<R>(d2, smi * o.z);
             ax += xd * g;
             ay += yd * g;
         }
@@ -282,7 +281,7 @@ extern "C" fa2_gpu *fa2_gpu_open(int device, int use_double, int n, int adjust,
         hs.check(g->d_rr.alloc(2 * nn * 4 * rs), "alloc");
         hs.check(g->d_ri.alloc(2 * nn), "alloc");
     } else {
-        hs.check(g->d_partial.alloc((size_t) slices * nn * 2 * rs), "alloc");
+        if (!g->use_double) hs.check(g->d_partial.alloc((size_t) slices * nn * 2 * rs), "alloc");
         hs.check(g->d_f64.alloc(nn), "alloc");
         hs.check(g->h_f64.alloc(nn), "alloc");
     }
@@ -301,6 +300,9 @@ static hipError_t launch_exact(fa2_gpu *g, double scaling)
     using V2 = typename Vec<R>::v2;
     const int n = g->n;
     const double *in = g->d_in.p;
+    if (std::is_same<R, double>::value) // the reference's operations, in order
+        return launch_exact_f64(g->stream, n, scaling, g->adjust, in, in + n, in + 2 * (size_t) n,
+                                g->adjust ? in + 3 * (size_t) n : nullptr, g->d_f64.p);
     k_pack_nodes<R><<<ceil_div(n, 256), 256, 0, g->stream>>>(
         n, in, in + n, in + 2 * (size_t) n, g->adjust ? in + 3 * (size_t) n : nullptr,
         (V4 *) g->d_p.p);
@@ -371,6 +373,10 @@ static hipError_t launch_bh(fa2_gpu *g, int nreg, double scaling, double theta)
         (V4 *) g->d_p.p);
     k_pack_regions<R><<<ceil_div(nreg, 256), 256, 0, g->stream>>>(
         nreg, g->d_reg.p, (V4 *) g->d_rr.p, g->d_ri.p);
+    if (std::is_same<R, double>::value) // the reference's operations, in order
+        return launch_bh_f64(g->stream, n, scaling, theta * theta, g->adjust, in, in + n,
+                             in + 2 * (size_t) n, g->adjust ? in + 3 * (size_t) n : nullptr,
+                             (const double4 *) g->d_rr.p, g->d_ri.p, (double2 *) g->d_f.p);
     const R th2 = (R) (theta * theta);
     if (g->adjust)
         k_bh_host_tree<R, true><<<ceil_div(n, kBhThreads), kBhThreads, 0, g->stream>>>(

@@ -93,6 +93,14 @@ int fa2_batch_stage_times(fa2_batch *b, double *times);
  * monopoles, regions opened, node pairs). */
 int fa2_batch_count_work(fa2_batch *b, unsigned long long out[3]);
 
+/* Replace the state of graph g between two runs: positions (x then y), the
+ * force of the previous iteration (x then y; NULL for zero, as at the
+ * start), the speed and the speed efficiency. With the positions alone
+ * (f_old NULL, speed = efficiency = 1) this restarts a layout from stored
+ * coordinates as the reference does; with the full state it resumes one. */
+int fa2_batch_set_state(fa2_batch *b, int g, const double *pos, const double *f_old,
+                        double speed, double speed_efficiency);
+
 /* The HIP stream of the batch (hipStream_t), for callers that time kernels
  * with events on it. */
 void *fa2_batch_stream(fa2_batch *b);

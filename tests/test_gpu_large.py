@@ -1,0 +1,293 @@
+"""Parity of the all-GPU path (fa2_full_run / fa2_batch_*) with the oracle.
+
+Gates (written in each test):
+  * Morton codes, sorted permutation, Karras topology: bit-exact against
+    the reference large path restated on the CPU, on identical float32
+    positions.
+  * forces of one iteration from identical state: normwise
+    max|dF| / max|F_ref| <= 1e-5 in single precision against the FP64
+    oracle evaluated on the same float32 positions, <= 1e-12 in double.
+  * trajectories: step by step over 10 iterations, then the quality ratio
+    (mean edge length / sd of the layout) within 2% of the CPU reference.
+Relational tests follow tests/testthat/test-large.R of the reference.
+"""
+import numpy as np
+import pytest
+
+import forceatlas2_rc_amd as F
+from forceatlas2_rc_amd import Batch, graphs
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def normwise(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def mean_rel(a, b):
+    return np.abs(a - b).mean() / np.abs(b).mean()
+
+
+def pa(n, m=2, seed=42):
+    return graphs.barabasi_albert(n, m, seed=seed)
+
+
+def start(n, seed=7, r=None):
+    r = 10 * np.sqrt(n) if r is None else r
+    return graphs.uniform_positions(n, -r, r, seed)
+
+
+def one_step(n, e, pos, precision="single", **kw):
+    g = dict(n=n, edges=e, init=pos, iterations=1, **kw)
+    with Batch([g], precision=precision) as b:
+        b.run(1)
+        b.sync()
+        return b.tree(0), b.forces(0), b.speed(0), b.positions(0)
+
+
+def oracle_kw(kw):
+    out = dict(theta=kw.get("theta", 1.2), linlog=kw.get("linlog", False),
+               outbound=kw.get("dissuade_hubs", False),
+               adjust=kw.get("adjust", False), strong=kw.get("strong", False),
+               scaling=kw.get("scaling", 2.0), gravity=kw.get("gravity", 1.0),
+               jt=kw.get("jitter_tolerance", 1.0))
+    if "size" in kw:
+        out["size"] = kw["size"]
+    if "fixed" in kw:
+        out["fixed"] = kw["fixed"]
+    return out
+
+
+@pytest.mark.parametrize("n,dup", [(2, 0), (3, 0), (700, 30), (3000, 0), (9000, 300)])
+def test_tree_is_bit_exact(n, dup):
+    """Codes, permutation and topology against the reference large path
+    (pipeline.cl:105-270) on the same float32 positions; coincident nodes
+    force equal Morton codes (test-large.R:121-129)."""
+    n, e = pa(n) if n > 2 else (n, np.array([[0, 1]]))
+    pos = start(n)
+    if dup:
+        pos[:dup] = pos[np.arange(dup) % 3]
+    t, *_ = one_step(n, e, pos)
+    p32 = pos.astype(np.float32)
+    ref = O.large_tree(p32[:, 0], p32[:, 1], "f")
+    for k in ("keys", "order", "child", "range", "parent"):
+        assert np.array_equal(t[k].astype(np.int64), ref[k].astype(np.int64)), k
+
+
+def test_tree_is_bit_exact_in_double():
+    n, e = pa(2000)
+    pos = start(n)
+    t, *_ = one_step(n, e, pos, precision="double")
+    ref = O.large_tree(pos[:, 0], pos[:, 1], "d")
+    for k in ("keys", "order", "child", "range", "parent"):
+        assert np.array_equal(t[k].astype(np.int64), ref[k].astype(np.int64)), k
+
+
+OPTS = [
+    dict(),
+    dict(linlog=True, dissuade_hubs=True),
+    dict(strong=True, gravity=0.3),
+    dict(adjust=True, size="rand"),
+    dict(adjust=True, size="rand", linlog=True, dissuade_hubs=True),
+    dict(fixed="alt", scaling=5.0, gravity=3.0),
+]
+
+
+def _opts(o, n):
+    o = dict(o)
+    if o.get("size") == "rand":
+        o["size"] = np.random.default_rng(1).uniform(0.5, 4, n)
+    if o.get("fixed") == "alt":
+        o["fixed"] = np.arange(n) % 2 == 0
+    return o
+
+
+@pytest.mark.parametrize("opts", OPTS, ids=lambda o: ",".join(o) or "default")
+@pytest.mark.parametrize("theta", [0.0, 1.2])
+def test_one_iteration_forces_single(opts, theta):
+    """Single precision against the FP64 oracle on the same float32
+    positions: normwise <= 1e-5."""
+    n, e = pa(4000, 3, seed=5)
+    pos = start(n, seed=6)
+    kw = _opts(opts, n)
+    kw["theta"] = theta
+    _, f, sp, _ = one_step(n, e, pos, **kw)
+    _, tr = O.large_run(n, e, pos, 1, mode="m", trace=True, **oracle_kw(kw))
+    assert normwise(f, tr["force"][0]) <= 1e-5
+    assert sp == pytest.approx(tr["speed"][0], rel=1e-5)
+
+
+@pytest.mark.parametrize("opts", OPTS[:4], ids=lambda o: ",".join(o) or "default")
+def test_one_iteration_forces_double(opts):
+    n, e = pa(3000, 3, seed=5)
+    pos = start(n, seed=6)
+    kw = _opts(opts, n)
+    _, f, sp, _ = one_step(n, e, pos, precision="double", **kw)
+    _, tr = O.large_run(n, e, pos, 1, mode="d", trace=True, **oracle_kw(kw))
+    assert normwise(f, tr["force"][0]) <= 1e-12
+    assert sp == pytest.approx(tr["speed"][0], rel=1e-12)
+
+
+def test_ten_iterations_step_by_step():
+    """The first 10 iterations, step by step from identical state: the
+    oracle's state before step t (positions, previous force, speed and
+    efficiency) is loaded into the device, one iteration runs, and its force
+    and displacement are compared with the oracle's: normwise <= 1e-5."""
+    n, e = pa(3000, 3, seed=8)
+    pos = start(n, seed=9)
+    _, tr = O.large_run(n, e, pos, 10, mode="m", theta=1.2, linlog=True,
+                        outbound=True, trace=True)
+    g = dict(n=n, edges=e, init=pos, theta=1.2, linlog=True, dissuade_hubs=True)
+    with Batch([g]) as b:
+        for t in range(10):
+            p0 = pos if t == 0 else tr["pos"][t - 1]
+            if t > 0:
+                sp, ef = tr["speed"][t - 1][2:]
+                b.set_state(0, p0, tr["force"][t - 1], sp, ef)
+            b.run(1)
+            b.sync()
+            assert normwise(b.forces(0), tr["force"][t]) <= 1e-5, t
+            disp = b.positions(0) - p0
+            assert normwise(disp, tr["pos"][t] - p0) <= 1e-5, t
+            assert b.speed(0)[2:] == pytest.approx(tr["speed"][t][2:], rel=1e-5)
+
+
+def test_double_without_approximation_is_the_exact_layout():
+    """test-large.R:10-38: theta = 0 in double equals the exact CPU layout
+    within 1e-8 after 5 iterations, with coincident nodes."""
+    n, e = pa(700, 2)
+    pos = start(n, r=50)
+    pos[:30] = pos[np.repeat(np.arange(3), 10)]
+    rng = np.random.default_rng(3)
+    w = rng.uniform(0.2, 3, len(e))
+    size = rng.uniform(0.5, 4, n)
+    for kw in (dict(), dict(scaling_ratio=5, gravity=3),
+               dict(linlog=True, strong_gravity=True),
+               dict(dissuade_hubs=True, gravity=0),
+               dict(weights=w, edge_weight_influence=0.5, jitter_tolerance=0.3),
+               dict(prevent_overlap=True, node_size=size),
+               dict(prevent_overlap=True, node_size=size, linlog=True,
+                    dissuade_hubs=True, weights=w),
+               dict(fixed=np.arange(n) % 2 == 0)):
+        got = F.layout_forceatlas2_large(n, e, 5, init=pos, theta=0.0,
+                                         precision="double", **kw)
+        okw = dict(scaling=kw.get("scaling_ratio", 2.0),
+                   gravity=kw.get("gravity", 1.0),
+                   strong=kw.get("strong_gravity", False),
+                   linlog=kw.get("linlog", False),
+                   outbound=kw.get("dissuade_hubs", False),
+                   adjust=kw.get("prevent_overlap", False),
+                   jt=kw.get("jitter_tolerance", 1.0), fixed=kw.get("fixed"),
+                   size=kw.get("node_size", 1.0))
+        if "weights" in kw:
+            okw["w"] = F.prepare_weights(len(e), kw["weights"],
+                                         kw.get("edge_weight_influence", 1.0))
+        want = O.layout(n, e, pos, 5, **okw)
+        assert mean_rel(got, want) < 1e-8, kw.keys()
+
+
+def test_theta_sweep():
+    """test-large.R:40-57."""
+    n, e = pa(3000)
+    pos = start(n, r=50)
+    exact = O.layout(n, e, pos, 1)
+
+    def err(theta, precision="double"):
+        xy = F.layout_forceatlas2_large(n, e, 1, init=pos, theta=theta,
+                                        precision=precision)
+        return np.abs(xy - exact).max() / np.ptp(exact)
+
+    assert err(0.1) < 1e-4
+    assert err(0.1) < err(0.5) < err(1.2) < 0.02
+    assert err(1.2, "single") < 0.02
+    assert err(0.0, "single") < 1e-3
+
+
+def test_reproducible_and_same_alone_as_in_a_batch():
+    """test-large.R:59-71, and batch composition invariance."""
+    n, e = pa(5000)
+    pos = start(n)
+    a = F.layout_forceatlas2_large(n, e, 30, init=pos)
+    b = F.layout_forceatlas2_large(n, e, 30, init=pos)
+    assert np.array_equal(a, b)
+    gs = []
+    for i, m in enumerate([300, 5000, 2049, 777]):
+        m, ee = pa(m, 2, seed=20 + i)
+        gs.append(dict(n=m, edges=ee, init=start(m, seed=i), iterations=25,
+                       linlog=bool(i % 2), theta=1.0 + 0.1 * i))
+    with Batch(gs) as bb:
+        bb.run(25)
+        bb.sync()
+        together = [bb.positions(i) for i in range(len(gs))]
+    for i, g in enumerate(gs):
+        with Batch([g]) as one:
+            one.run(25)
+            one.sync()
+            assert np.array_equal(one.positions(0), together[i])
+
+
+def test_layout_quality_matches_cpu():
+    """test-large.R:73-94 (5%) and the 2% edge-length/stress gate: islands
+    4 x 100, 300 iterations."""
+    n, e = graphs.islands(4, 100, 0.1, 2, seed=1)
+    pos = start(n, r=50)
+
+    def edge_length(xy):
+        d = np.hypot(*(xy[e[:, 0]] - xy[e[:, 1]]).T)
+        return d.mean() / np.std(xy, ddof=1)
+
+    cpu = O.layout(n, e, pos, 300, barnes_hut=True)
+    gpu = F.layout_forceatlas2_large(n, e, 300, init=pos)
+    assert np.all(np.isfinite(gpu))
+    assert edge_length(gpu) == pytest.approx(edge_length(cpu), rel=0.02)
+    assert np.std(gpu, ddof=1) == pytest.approx(np.std(cpu, ddof=1), rel=0.05)
+
+
+def test_cliques_separate():
+    e = [(i, j) for b in (0, 10) for i in range(b, b + 10) for j in range(i + 1, b + 10)]
+    e.append((0, 10))
+    xy = F.layout_forceatlas2_large(20, np.array(e), 500, rng=1)
+    d = np.hypot(*(xy[:, None] - xy[None]).transpose(2, 0, 1))
+    assert np.all(np.isfinite(xy))
+    assert d[:10, :10].max() < d[1:10, 11:20].min()
+
+
+def test_degenerate_graphs():
+    """test-large.R:96-133."""
+    assert F.layout_forceatlas2_large(0, np.zeros((0, 2), int), 10).shape == (0, 2)
+    assert F.layout_forceatlas2_large(1, np.zeros((0, 2), int), 10, rng=1).shape == (1, 2)
+    for n in range(2, 13):
+        ring = np.array([(i, (i + 1) % n) for i in range(n)])
+        xy = F.layout_forceatlas2_large(n, ring, 50, rng=n)
+        assert np.all(np.isfinite(xy))
+        d = np.hypot(*(xy[:, None] - xy[None]).transpose(2, 0, 1))
+        assert d[np.triu_indices(n, 1)].min() > 0
+    assert np.all(np.isfinite(F.layout_forceatlas2_large(300, np.zeros((0, 2), int), 100, rng=1)))
+    g = np.array([[0, 0], [0, 1], [0, 1], [1, 2]])
+    assert np.all(np.isfinite(F.layout_forceatlas2_large(5, g, 100, rng=1)))
+    n, e = pa(500)
+    zero = np.zeros((n, 2))
+    assert np.array_equal(F.layout_forceatlas2_large(n, e, 20, init=zero), zero)
+    line = np.stack([np.arange(1, n + 1), np.zeros(n)], 1).astype(float)
+    assert np.all(np.isfinite(F.layout_forceatlas2_large(n, e, 20, init=line)))
+    pos = start(n, r=50)
+    pos[:300] = 1.0
+    assert np.all(np.isfinite(F.layout_forceatlas2_large(n, e, 20, init=pos)))
+    assert np.allclose(F.layout_forceatlas2_large(n, e, 0, init=pos), pos)
+
+
+def test_errors():
+    n, e = pa(60)
+    with pytest.raises(ValueError, match="device"):
+        F.layout_forceatlas2_large(n, e, 5, device=len(F.gpu_devices()) + 1)
+    with pytest.raises(ValueError, match="theta"):
+        F.layout_forceatlas2_large(n, e, 5, theta=-1)
+
+
+def test_profile_times_every_phase():
+    n, e = pa(3000)
+    xy, t = F.layout_forceatlas2_large(n, e, 5, rng=1, timings=True)
+    assert t["total"] > 0
+    for k in ("codes", "sort", "tree", "repulsion", "attraction", "move"):
+        assert t[k] > 0, k

@@ -177,3 +177,67 @@ def test_topology_and_walk(dup):
             elif v[0] == "b":
                 seen.extend(range(v[1], v[2] + 1))
         assert sorted(seen) == list(range(n))
+
+
+def wave_visits(T, g, lanes, accept):
+    """The wave walk of k_repulse: the union of the lanes' visits in
+    depth-first order, each lane taking part only where its own traversal
+    goes. Returns the sequence each lane took part in."""
+    n, base = T["ns"][g], int(T["noff"][g])
+    gend = base + n
+    io = int(T["ioff"][g])
+    done = {k: -(1 << 31) + 1 for k in lanes}
+    seq = {k: [] for k in lanes}
+    c = io
+    steps = 0
+    while c != END:
+        steps += 1
+        assert steps <= 2 * n
+        if c >= 0:
+            left, skip, first, last = T["tl"][c]
+            act = [k for k in lanes if first > done[k]]
+            acc = [k for k in act if accept(k, c)]
+            opn = [k for k in act if k not in acc]
+            for k in act:
+                seq[k].append(("r", c - io))
+            for k in acc:
+                done[k] = last
+            if not opn:
+                c = skip
+                continue
+            if last - first >= 8:
+                c = left
+                continue
+            for k in opn:
+                seq[k].append(("b", first - base, last - base))
+                done[k] = last
+            c = skip
+        else:
+            p = ~c
+            for k in lanes:
+                if p > done[k]:
+                    seq[k].append(("l", p - base))
+                    done[k] = p
+            c = END if p == gend - 1 else T["escape"][p - g]
+    return seq
+
+
+@pytest.mark.parametrize("dup", [False, True])
+def test_wave_walk_gives_each_lane_its_own_traversal(dup):
+    rng = np.random.default_rng(11 + dup)
+    sizes = [3, 70, 300]
+    keys = [_keys(rng, n, dup) for n in sizes]
+    T = engine_tree(keys)
+    for g, n in enumerate(sizes):
+        child, rng_, parent = O.karras(keys[g])
+        io = int(T["ioff"][g])
+        # each lane accepts a region with its own pseudo-random rule,
+        # correlated with its position like a real distance test
+        salt = rng.integers(0, 1 << 30, n)
+        acc = lambda k, r: ((int(salt[k]) // 7 + int(r) * 2654435761) >> 3) % 3 == 0 \
+            and abs(rng_[2 * r] - k) > 4
+        lanes = list(range(min(n, 64)))
+        got = wave_visits(T, g, lanes, lambda k, c: acc(k, c - io))
+        for k in lanes:
+            want = stack_visits(n, child, rng_, k, lambda r: acc(k, r))
+            assert got[k] == want
