@@ -81,6 +81,9 @@ def main():
     ap.add_argument("--precision", default="single")
     ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="first run one device-to-device copy of this many "
+                         "bytes (a known-size reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU "
@@ -90,6 +93,13 @@ def main():
 
     from forceatlas2_rc_amd import Batch
     from forceatlas2_rc_amd.api import prepare_large
+
+    if args.copy_bytes:
+        import torch
+        src = torch.empty(args.copy_bytes // 4, dtype=torch.float32, device="cuda")
+        dst = src.clone()
+        torch.cuda.synchronize()
+        del src, dst
 
     t0 = time.time()
     n, e = make_graph(args.scale, args.edges, args.seed)

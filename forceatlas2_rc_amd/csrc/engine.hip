@@ -238,15 +238,52 @@ k_sort_rowscan(int rows, const GraphDesc *__restrict__ gd, uint32_t *__restrict_
     if (lane == 0) rowtot[r] = carry;
 }
 
+// The same with one block per row, for graphs of more than 64 tiles.
+__global__ void __launch_bounds__(256)
+k_sort_rowscan_block(const GraphDesc *__restrict__ gd, uint32_t *__restrict__ hist,
+                     uint32_t *__restrict__ rowtot)
+{
+    __shared__ uint32_t wsum[4];
+    const int r = blockIdx.x;
+    const int g = r >> 8, d = r & 255;
+    const GraphDesc G = gd[g];
+    uint32_t *row = hist + (size_t) G.toff * 256 + (size_t) d * G.ntiles;
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (int b = 0; b < G.ntiles; b += 256) {
+        const int j = b + (int) threadIdx.x;
+        const uint32_t v = j < G.ntiles ? row[j] : 0u;
+        uint32_t s = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(s, o, 64);
+            if (lane >= o) s += u;
+        }
+        if (lane == 63) wsum[w] = s;
+        __syncthreads();
+        uint32_t before = carry;
+        for (int k = 0; k < w; k++) before += wsum[k];
+        const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (j < G.ntiles) row[j] = before + s - v;
+        __syncthreads();
+        carry += total;
+    }
+    if (threadIdx.x == 0) rowtot[r] = carry;
+}
+
 // Stable scatter of one tile. Wave w ranks its 512 keys in 8 rounds of 64
 // (ballot match on the 8 digit bits), then the waves are combined in order:
 // the rank of a key counts the equal digits before it in the tile. Keys go
 // through LDS so that the global writes of one digit are contiguous.
+// The last pass (GATHER) also writes psort[k] = node4[order[k]], the
+// (x, y, mass, radius) of the nodes in sorted order (pipeline.cl:195-207).
+template <typename V4, bool GATHER>
 __global__ void __launch_bounds__(kSortThreads)
 k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
                  const uint32_t *__restrict__ key_in, const uint32_t *__restrict__ val_in,
                  int shift, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ rowtot,
-                 uint32_t *__restrict__ key_out, uint32_t *__restrict__ val_out)
+                 uint32_t *__restrict__ key_out, uint32_t *__restrict__ val_out,
+                 const V4 *__restrict__ node4, V4 *__restrict__ psort)
 {
     __shared__ uint32_t wcnt[4][256];
     __shared__ int goff[256];
@@ -336,8 +373,10 @@ k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
         const int d = (int) ((k >> shift) & 255u);
         const int dst = goff[d] + p;
         if (dst < G.noff || dst >= G.noff + G.n) continue; // never taken
+        const uint32_t v = sval[p];
         key_out[dst] = k;
-        val_out[dst] = sval[p];
+        val_out[dst] = v;
+        if (GATHER) psort[dst] = node4[v];
     }
 }
 
@@ -345,16 +384,16 @@ k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
 // Tree: gather, Karras radix tree, bottom-up summaries (pipeline.cl:195-323)
 // ---------------------------------------------------------------------------
 
+// node4[i] = (x, y, mass, radius), after positions were replaced.
 template <typename R>
-__global__ void k_gather(int N, const uint32_t *__restrict__ order,
-                         const typename Vec<R>::v2 *__restrict__ pos, const R *__restrict__ mass,
-                         const R *__restrict__ size, typename Vec<R>::v4 *__restrict__ psort)
+__global__ void k_pack_node4(int N, const typename Vec<R>::v2 *__restrict__ pos,
+                             const R *__restrict__ mass, const R *__restrict__ size,
+                             typename Vec<R>::v4 *__restrict__ node4)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
-    const uint32_t i = order[k];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
     const typename Vec<R>::v2 p = pos[i];
-    psort[k] = mk4<R>(p.x, p.y, mass[i], size[i]);
+    node4[i] = mk4<R>(p.x, p.y, mass[i], size[i]);
 }
 
 // Common prefix of sorted keys a and b of one graph (local positions),
@@ -775,33 +814,58 @@ k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
 constexpr int kChunk = 1024;
 
 template <typename R>
+__device__ inline void attract_term(typename Vec<R>::v2 pi, typename Vec<R>::v2 pj, R c, R si,
+                                    R sj, bool linlog, bool adjust, KahanSum2<R> &sum)
+{
+    const R xd = pi.x - pj.x, yd = pi.y - pj.y;
+    R f = -c;
+    if (adjust) {
+        // the gap between the discs, in FP64 as for the repulsion: the edge
+        // only pulls while it is positive
+        const double dx = (double) xd, dy = (double) yd;
+        const double gap = __builtin_sqrt(dx * dx + dy * dy) - ((double) si + (double) sj);
+        if (!(gap > 0.0)) return;
+        if (linlog) {
+            const R dist = (R) gap;
+            f *= flog1p_r(dist) / dist;
+        }
+    } else if (linlog) {
+        const R dist = fsqrt_r(xd * xd + yd * yd);
+        if (!(dist > (R) 0)) return;
+        f *= flog1p_r(dist) / dist;
+    }
+    sum.add(xd * f, yd * f);
+}
+
+// The incidences e0, e0 + step, ... < e1 of one node, in that order; the
+// loads of four incidences are issued before their arithmetic.
+template <typename R>
 __device__ inline void attract_terms(int e0, int e1, int step, bool linlog, bool adjust,
                                      typename Vec<R>::v2 pi, R si,
                                      const typename Vec<R>::v2 *__restrict__ pos,
                                      const R *__restrict__ size, const int *__restrict__ inc_node,
                                      const R *__restrict__ inc_coef, KahanSum2<R> &sum)
 {
-    for (int e = e0; e < e1; e += step) {
-        const int j = inc_node[e];
-        const typename Vec<R>::v2 pj = pos[j];
-        const R xd = pi.x - pj.x, yd = pi.y - pj.y;
-        R f = -inc_coef[e];
-        if (adjust) {
-            // the gap between the discs, in FP64 as for the repulsion: the
-            // edge only pulls while it is positive
-            const double dx = (double) xd, dy = (double) yd;
-            const double gap = __builtin_sqrt(dx * dx + dy * dy) - ((double) si + (double) size[j]);
-            if (!(gap > 0.0)) continue;
-            if (linlog) {
-                const R dist = (R) gap;
-                f *= flog1p_r(dist) / dist;
-            }
-        } else if (linlog) {
-            const R dist = fsqrt_r(xd * xd + yd * yd);
-            if (!(dist > (R) 0)) continue;
-            f *= flog1p_r(dist) / dist;
+    constexpr int U = 4;
+    for (int e = e0; e < e1; e += U * step) {
+        int j[U];
+        R c[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int q = e + u * step;
+            j[u] = q < e1 ? inc_node[q] : -1;
+            c[u] = q < e1 ? inc_coef[q] : (R) 0;
         }
-        sum.add(xd * f, yd * f);
+        typename Vec<R>::v2 pj[U];
+        R sj[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            pj[u] = j[u] >= 0 ? pos[j[u]] : pi;
+            sj[u] = (adjust && j[u] >= 0) ? size[j[u]] : (R) 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (j[u] >= 0) attract_term<R>(pi, pj[u], c[u], si, sj[u], linlog, adjust, sum);
     }
 }
 
@@ -1019,6 +1083,7 @@ k_apply(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
         const GraphState *__restrict__ gs, const R *__restrict__ mass,
         const uint8_t *__restrict__ fixed, const double2 *__restrict__ F,
         const double2 *__restrict__ Fold, typename Vec<R>::v2 *__restrict__ pos,
+        const R *__restrict__ size, typename Vec<R>::v4 *__restrict__ node4,
         double2 *__restrict__ acc, typename Vec<R>::v4 *__restrict__ tbox)
 {
     __shared__ R lds[16];
@@ -1059,6 +1124,7 @@ k_apply(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
                 p.y += (R) ((double) f.y * factor);
             }
             pos[i] = p;
+            node4[i] = mk4<R>(p.x, p.y, mass[i], size[i]);
         }
         a = min(a, p.x);
         b = min(b, p.y);
@@ -1092,6 +1158,8 @@ template <typename R> struct EngineImpl final : Engine {
     DevBuf<Tile> d_tiles;
     DevBuf<int> d_gid, d_gi;
     DevBuf<V2> d_pos;
+    DevBuf<V4> d_node4; // (x, y, mass, radius) in the original order
+    int max_tiles = 1;
     DevBuf<double2> d_F[2]; // forces are kept in FP64: the swing is their difference
     DevBuf<double2> d_acc;
     DevBuf<R> d_mass, d_size, d_coef;
@@ -1206,6 +1274,7 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     }
     T = toff;
     NS = (int) h_supers.size();
+    for (const GraphDesc &D : h_gd) max_tiles = std::max(max_tiles, D.ntiles);
 
     // packed host arrays
     const size_t nn = (size_t) N;
@@ -1273,6 +1342,7 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     hs.check(d_gid.alloc(nn), "alloc");
     hs.check(d_gi.alloc(NI), "alloc");
     hs.check(d_pos.alloc(nn), "alloc");
+    hs.check(d_node4.alloc(nn), "alloc");
     hs.check(d_F[0].alloc(nn), "alloc");
     hs.check(d_F[1].alloc(nn), "alloc");
     hs.check(d_acc.alloc(Acc<R>::separate ? nn : 1), "alloc");
@@ -1323,6 +1393,9 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     up(d_gid.p, gid.data(), nn * sizeof(int));
     if (NI > 0) up(d_gi.p, gi.data(), (size_t) NI * sizeof(int));
     up(d_pos.p, pos.data(), nn * sizeof(V2));
+    std::vector<V4> node4(nn);
+    for (size_t i = 0; i < nn; i++) node4[i] = mk4<R>(pos[i].x, pos[i].y, mass[i], size[i]);
+    up(d_node4.p, node4.data(), nn * sizeof(V4));
     if (Acc<R>::separate) up(d_acc.p, acc.data(), nn * sizeof(double2));
     up(d_mass.p, mass.data(), nn * sizeof(R));
     up(d_size.p, size.data(), nn * sizeof(R));
@@ -1369,20 +1442,27 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
     k_graph_bbox<R><<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_tbox.p, d_gbox.p);
     k_morton<R><<<T, 256, 0, stream>>>(d_tiles.p, d_pos.p, d_gbox.p, d_keyA.p, d_valA.p);
     mark();
-    // 3: sort, A -> B -> A -> B -> A
+    // 3: sort, A -> B -> A -> B -> A; the last pass gathers psort
     for (int pass = 0; pass < 4; pass++) {
         uint32_t *ki = pass % 2 ? d_keyB.p : d_keyA.p, *vi = pass % 2 ? d_valB.p : d_valA.p;
         uint32_t *ko = pass % 2 ? d_keyA.p : d_keyB.p, *vo = pass % 2 ? d_valA.p : d_valB.p;
         k_sort_upsweep<<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, ki, 8 * pass, d_hist.p);
-        k_sort_rowscan<<<ceil_div(256 * (size_t) G, 4), 256, 0, stream>>>(256 * G, d_gd.p, d_hist.p,
-                                                                          d_rowtot.p);
-        k_sort_downsweep<<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, ki, vi, 8 * pass,
-                                                         d_hist.p, d_rowtot.p, ko, vo);
+        if (max_tiles > 64)
+            k_sort_rowscan_block<<<256 * G, 256, 0, stream>>>(d_gd.p, d_hist.p, d_rowtot.p);
+        else
+            k_sort_rowscan<<<ceil_div(256 * (size_t) G, 4), 256, 0, stream>>>(256 * G, d_gd.p,
+                                                                              d_hist.p, d_rowtot.p);
+        if (pass < 3)
+            k_sort_downsweep<V4, false><<<T, kSortThreads, 0, stream>>>(
+                d_tiles.p, d_gd.p, ki, vi, 8 * pass, d_hist.p, d_rowtot.p, ko, vo, nullptr,
+                nullptr);
+        else
+            k_sort_downsweep<V4, true><<<T, kSortThreads, 0, stream>>>(
+                d_tiles.p, d_gd.p, ki, vi, 8 * pass, d_hist.p, d_rowtot.p, ko, vo, d_node4.p,
+                d_psort.p);
     }
     mark();
     // 4: tree
-    k_gather<R><<<ceil_div(N, 256), 256, 0, stream>>>(N, d_valA.p, d_pos.p, d_mass.p, d_size.p,
-                                                      d_psort.p);
     if (NI > 0) {
         k_karras<R><<<ceil_div(NI, 256), 256, 0, stream>>>(
             NI, d_gi.p, d_gd.p, d_keyA.p, d_child.p, d_tn.p, d_parent_leaf.p, d_parent_int.p,
@@ -1436,7 +1516,7 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
     k_swing<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_mass.p, d_fixed.p, Fc, Fo, d_part.p);
     k_speed<<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_part.p, d_gs.p);
     k_apply<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, d_mass.p, d_fixed.p, Fc, Fo,
-                                      d_pos.p, d_acc.p, d_tbox.p);
+                                      d_pos.p, d_size.p, d_node4.p, d_acc.p, d_tbox.p);
     mark();
     parity = 1 - parity;
 }
@@ -1711,6 +1791,8 @@ int EngineImpl<R>::set_state(int graph, const double *pos, const double *fold, d
                       hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         k_tile_bbox<R><<<T, 256, 0, stream>>>(d_tiles.p, d_pos.p, d_tbox.p);
+        k_pack_node4<R><<<ceil_div(N, 256), 256, 0, stream>>>(N, d_pos.p, d_mass.p, d_size.p,
+                                                              d_node4.p);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream);

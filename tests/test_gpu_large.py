@@ -94,10 +94,12 @@ OPTS = [
 ]
 
 
-def _opts(o, n):
+def _opts(o, n, single=True):
     o = dict(o)
     if o.get("size") == "rand":
-        o["size"] = np.random.default_rng(1).uniform(0.5, 4, n)
+        # the device holds sizes as float32: the oracle gets the same values
+        size = np.random.default_rng(1).uniform(0.5, 4, n)
+        o["size"] = size.astype(np.float32).astype(np.float64) if single else size
     if o.get("fixed") == "alt":
         o["fixed"] = np.arange(n) % 2 == 0
     return o
@@ -122,7 +124,7 @@ def test_one_iteration_forces_single(opts, theta):
 def test_one_iteration_forces_double(opts):
     n, e = pa(3000, 3, seed=5)
     pos = start(n, seed=6)
-    kw = _opts(opts, n)
+    kw = _opts(opts, n, single=False)
     _, f, sp, _ = one_step(n, e, pos, precision="double", **kw)
     _, tr = O.large_run(n, e, pos, 1, mode="d", trace=True, **oracle_kw(kw))
     assert normwise(f, tr["force"][0]) <= 1e-12
