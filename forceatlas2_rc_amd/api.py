@@ -395,6 +395,17 @@ class Batch:
         return dict(keys=ks, order=order, child=child, range=rng_,
                     parent=parent)
 
+    def regions(self, g=0):
+        """Traversal records of graph g's regions in the reference numbering:
+        an (n-1, 4) array of (centre x, centre y, mass, size^2) and the skip
+        link of each region (region 0..n-2, leaf n-1.., -1 at the end)."""
+        n = self.n[g]
+        rec = np.empty(4 * (n - 1))
+        skip = np.empty(n - 1, np.int32)
+        if lib.fa2_batch_regions(self.h, g, _ptr(rec), _ptr(skip)) != 0:
+            raise GPUError(last_error())
+        return rec.reshape(n - 1, 4), skip
+
     def stage_times(self):
         out = np.zeros(FA2_FULL_NTIMES)
         lib.fa2_batch_stage_times(self.h, _ptr(out))

@@ -90,6 +90,12 @@ extern "C" int fa2_batch_tree(fa2_batch *b, int g, uint32_t *keys_sorted, uint32
     return b->e->tree(g, keys_sorted, order, child, range, parent);
 }
 
+extern "C" int fa2_batch_regions(fa2_batch *b, int g, double *rec, int *skip)
+{
+    if (!b) return 1;
+    return b->e->regions(g, rec, skip);
+}
+
 extern "C" int fa2_batch_stage_times(fa2_batch *b, double *times)
 {
     if (!b) return 1;

@@ -34,6 +34,11 @@ struct Engine {
     virtual int speed_state(int graph, double out[4]) = 0;
     virtual int tree(int graph, uint32_t *keys_sorted, uint32_t *order, int *child, int *range,
                      int *parent) = 0;
+    // Traversal records of the regions of one graph's tree, in the
+    // reference numbering: (centre x, centre y, mass, size^2) and the skip
+    // link (the region or leaf visited after the region's subtree, -1 at the
+    // end). Either pointer may be NULL.
+    virtual int regions(int graph, double *rec, int *skip) = 0;
     virtual int stage_times(double *t) = 0;
     virtual int graph_status(int graph) = 0;
     // Work of the repulsion on the current tree: regions accepted, regions

@@ -471,6 +471,29 @@ __device__ inline Summ<R> leaf_summ(const typename Vec<R>::v4 p)
 // the region, as exact arithmetic guarantees: rounded, it can fall an ulp
 // away from a cluster of coincident nodes, which would then see their own
 // region as a far monopole.
+template <typename R> __device__ inline Summ<R> merge_summ(const Summ<R> &a, const Summ<R> &b)
+{
+    Summ<R> r;
+    r.x0 = min(a.x0, b.x0);
+    r.y0 = min(a.y0, b.y0);
+    r.x1 = max(a.x1, b.x1);
+    r.y1 = max(a.y1, b.y1);
+    r.m = a.m + b.m;
+    r.mx = a.mx + b.mx;
+    r.my = a.my + b.my;
+    return r;
+}
+
+// (cx, cy, M, size^2) of a region (pipeline.cl:312-323), the centre clamped
+// into the bounding box.
+template <typename R> __device__ inline typename Vec<R>::v4 region_record(const Summ<R> &r)
+{
+    const R w = r.x1 - r.x0, h = r.y1 - r.y0;
+    const R cx = min(max(r.mx / r.m, r.x0), r.x1);
+    const R cy = min(max(r.my / r.m, r.y0), r.y1);
+    return mk4<R>(cx, cy, r.m, (R) 2 * (w * w + h * h));
+}
+
 template <typename R>
 __device__ inline void merge_region(int t, int g, int gend,
                                     const typename Vec<R>::v4 *__restrict__ psort,
@@ -492,20 +515,10 @@ __device__ inline void merge_region(int t, int g, int gend,
         const typename Vec<R>::v4 u = nd[2 * c.y], v = nd[2 * c.y + 1];
         b = Summ<R>{u.x, u.y, u.z, u.w, v.x, v.y, v.z};
     }
-    Summ<R> r;
-    r.x0 = min(a.x0, b.x0);
-    r.y0 = min(a.y0, b.y0);
-    r.x1 = max(a.x1, b.x1);
-    r.y1 = max(a.y1, b.y1);
-    r.m = a.m + b.m;
-    r.mx = a.mx + b.mx;
-    r.my = a.my + b.my;
+    const Summ<R> r = merge_summ<R>(a, b);
     nd[2 * t] = mk4<R>(r.x0, r.y0, r.x1, r.y1);
     nd[2 * t + 1] = mk4<R>(r.m, r.mx, r.my, (R) 0);
-    const R w = r.x1 - r.x0, h = r.y1 - r.y0;
-    const R cx = min(max(r.mx / r.m, r.x0), r.x1);
-    const R cy = min(max(r.my / r.m, r.y0), r.y1);
-    tn[t].c = mk4<R>(cx, cy, r.m, (R) 2 * (w * w + h * h));
+    tn[t].c = region_record<R>(r);
     const int last = tn[t].l.w;
     tn[t].l.y = last == gend - 1 ? kEnd : escape[last - g];
 }
@@ -542,6 +555,19 @@ __device__ inline void push_ready(const GraphDesc &D, int t, int first, int last
     }
 }
 
+// Level 1 in LDS. The workgroup of a block of sorted positions [lo, hi)
+// (a tile in single precision, half a tile in double, for the LDS budget)
+// first stages the block's leaf records and the links of the regions that
+// may lie inside it (internal ids tbase .. tbase + (hi - lo) - 1); a region
+// that leaves the block gets an arrival count that starts at kOutside, so
+// that one LDS atomic both classifies a region and counts its arrivals. The
+// climb then reads and writes summaries in LDS only. Written to HBM: the
+// traversal record of every region merged here, and the summaries of those
+// whose parent leaves the block (read by the next levels).
+constexpr int kOutside = 1 << 24;
+
+template <typename R> constexpr int agg_block() { return sizeof(R) == 4 ? kTile : kTile / 2; }
+
 template <typename R>
 __global__ void __launch_bounds__(1024)
 k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
@@ -550,30 +576,99 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
                  const int *__restrict__ escape, int NS, int N, int *gvis, int *ready, int *rcnt,
                  typename Vec<R>::v4 *nd, TNode<R> *tn)
 {
-    __shared__ int lvis[kTile];
-    const Tile T = tiles[blockIdx.x];
-    const int g = T.g, lo = T.lo, hi = T.hi;
+    using V4 = typename Vec<R>::v4;
+    constexpr int B = agg_block<R>();
+    constexpr int kParts = kTile / B;
+    __shared__ V4 s_leaf[B];
+    __shared__ int2 s_child[B];
+    __shared__ int s_parent[B];
+    __shared__ int s_vis[B];
+    __shared__ R s_sum[7][B]; // x0, y0, x1, y1, m, mx, my
+    const Tile T = tiles[blockIdx.x / kParts];
+    const int g = T.g;
+    const int lo = T.lo + (int) (blockIdx.x % kParts) * B;
+    const int hi = min(T.hi, lo + B);
+    if (lo >= hi) return;
     const GraphDesc D = gd[g];
     const int gend = D.noff + D.n;
-    const int tbase = lo - g; // regions inside the tile are numbered tbase..
-    for (int i = threadIdx.x; i < kTile; i += blockDim.x) lvis[i] = 0;
+    const int tbase = lo - g; // the regions that may lie inside: tbase + [0, hi - lo)
+    // internal ids stop at noff + n - 1 - g - 1: the last position of a
+    // graph has no region of its own index
+    const int nslots = (hi == gend) ? hi - lo - 1 : hi - lo;
+    for (int q = threadIdx.x; q < hi - lo; q += blockDim.x) {
+        s_leaf[q] = psort[lo + q];
+        int vis = kOutside;
+        if (q < nslots) {
+            const int t = tbase + q;
+            const int4 l = tn[t].l;
+            if (l.z >= lo && l.w < hi) vis = 0;
+            s_child[q] = child[t];
+            s_parent[q] = parent_int[t];
+        }
+        s_vis[q] = vis;
+    }
     __syncthreads();
     for (int k = lo + (int) threadIdx.x; k < hi; k += blockDim.x) {
         int t = parent_leaf[k];
+        Summ<R> r = leaf_summ<R>(s_leaf[k - lo]);
         // a root-to-leaf path has at most 64 regions (32 key bits, then up to
         // 32 bits of position); the bound only guards against a corrupt tree
         for (int depth = 0; t >= 0 && depth < 128; depth++) {
-            const int4 l = tn[t].l;
-            if (l.z < lo || l.w >= hi) { // the region spans several tiles
+            const int q = t - tbase;
+            const int old = (q >= 0 && q < nslots)
+                                ? __hip_atomic_fetch_add(&s_vis[q], 1, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP)
+                                : kOutside;
+            if (old >= kOutside) { // the region leaves the block: the next levels
+                const int4 l = tn[t].l;
                 if (atomicAdd(&gvis[t], 1) == 1) push_ready(D, t, l.z, l.w, NS, N, ready, rcnt);
                 break;
             }
-            const int old = __hip_atomic_fetch_add(&lvis[t - tbase], 1, __ATOMIC_ACQ_REL,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0) break;
-            merge_region<R>(t, g, gend, psort, child, escape, nd, tn);
-            t = parent_int[t];
+            if (old == 0) break; // the other child is not summarised yet
+            const int2 c = s_child[q];
+            Summ<R> a, b;
+            if (c.x < 0) {
+                a = leaf_summ<R>(s_leaf[~c.x - lo]);
+            } else {
+                const int u = c.x - tbase;
+                a = Summ<R>{s_sum[0][u], s_sum[1][u], s_sum[2][u], s_sum[3][u],
+                            s_sum[4][u], s_sum[5][u], s_sum[6][u]};
+            }
+            if (c.y < 0) {
+                b = leaf_summ<R>(s_leaf[~c.y - lo]);
+            } else {
+                const int u = c.y - tbase;
+                b = Summ<R>{s_sum[0][u], s_sum[1][u], s_sum[2][u], s_sum[3][u],
+                            s_sum[4][u], s_sum[5][u], s_sum[6][u]};
+            }
+            r = merge_summ<R>(a, b);
+            s_sum[0][q] = r.x0;
+            s_sum[1][q] = r.y0;
+            s_sum[2][q] = r.x1;
+            s_sum[3][q] = r.y1;
+            s_sum[4][q] = r.m;
+            s_sum[5][q] = r.mx;
+            s_sum[6][q] = r.my;
+            tn[t].c = region_record<R>(r);
+            const int p = s_parent[q];
+            if (p >= 0) {
+                const int pq = p - tbase;
+                if (pq < 0 || pq >= nslots || s_vis[pq] >= kOutside) {
+                    // the parent leaves the block: its merge reads this summary
+                    nd[2 * t] = mk4<R>(r.x0, r.y0, r.x1, r.y1);
+                    nd[2 * t + 1] = mk4<R>(r.m, r.mx, r.my, (R) 0);
+                }
+            }
+            t = p;
         }
+    }
+    __syncthreads();
+    // the skip links of the regions merged here
+    for (int q = threadIdx.x; q < nslots; q += blockDim.x) {
+        if (s_vis[q] >= kOutside) continue;
+        const int t = tbase + q;
+        const int last = tn[t].l.w;
+        tn[t].l.y = last == gend - 1 ? kEnd : escape[last - g];
     }
 }
 
@@ -651,44 +746,46 @@ k_aggregate_top(const GraphDesc *__restrict__ gd, const typename Vec<R>::v4 *__r
 // and accumulates the same terms in the same order.
 // ---------------------------------------------------------------------------
 
-// Per-lane force of the traversal: each term is computed in the working
-// precision, the running sum is kept in FP64 (two adds and two conversions
-// per term), so long sums with cancellation keep the precision of the terms.
-template <typename R> struct Sum2 {
+// Per-lane force of the traversal. A region or node o contributes
+// scaling * m_i * M_o / d^2 * (xd, yd); the common factor scaling * m_i is
+// applied once at the end. The weight w = M_o / d^2 and the offsets are
+// computed in the working precision; each term xd * w is formed exactly in
+// FP64 (a product of two FP32 values fits in 53 bits) by a fused multiply-add
+// into the FP64 sum. A lane's sum therefore depends on its own terms only,
+// not on the other lanes of its wave: a graph gives the same result alone
+// and inside any batch.
+template <typename R> struct ForceSum {
     double x = 0.0, y = 0.0;
-    __device__ inline void add(R tx, R ty)
+    __device__ inline void add(R xd, R yd, R w)
     {
-        x += (double) tx;
-        y += (double) ty;
+        const double wd = (double) w;
+        x = __builtin_fma((double) xd, wd, x);
+        y = __builtin_fma((double) yd, wd, y);
     }
 };
 
-// The force of node o on this lane's node, added to sum when `on`. With
-// node sizes (prevent_overlap) the term is computed in FP64 from the FP32
-// inputs: the force jumps by orders of magnitude as two discs touch, and the
-// terms of overlapping discs are large and cancel.
-template <typename R, bool ADJUST>
-__device__ inline void lane_pair(typename Vec<R>::v4 me, typename Vec<R>::v4 o, R smi,
-                                 double sscale, bool on, Sum2<R> &sum)
+
+// With node sizes (prevent_overlap) a node pair's term is computed in FP64
+// from the FP32 inputs: the force jumps by orders of magnitude as two discs
+// touch, and the terms of overlapping discs are large and cancel. The term
+// goes straight into the FP64 total, without the factor scaling * m_i.
+template <typename R>
+__device__ inline void lane_pair_adjust(typename Vec<R>::v4 me, typename Vec<R>::v4 o,
+                                        ForceSum<R> &sum, bool on)
 {
-    const R xd = me.x - o.x, yd = me.y - o.y;
-    if (ADJUST) {
-        const double dx = (double) xd, dy = (double) yd;
-        const double d = __builtin_sqrt(dx * dx + dy * dy) - ((double) me.w + (double) o.w);
-        const double smm = sscale * (double) o.z;
-        double f = d > 0.0 ? smm / (d * d) : (d < 0.0 ? 100.0 * smm : 0.0);
-        f = on ? f : 0.0;
-        sum.x += dx * f;
-        sum.y += dy * f;
-    } else {
-        R f = pair_factor<R>(xd * xd + yd * yd, smi * o.z);
-        f = on ? f : (R) 0;
-        sum.add(xd * f, yd * f);
-    }
+    const double dx = (double) (me.x - o.x), dy = (double) (me.y - o.y);
+    const double d = __builtin_sqrt(dx * dx + dy * dy) - ((double) me.w + (double) o.w);
+    const double mo = (double) o.z;
+    double f = d > 0.0 ? mo / (d * d) : (d < 0.0 ? 100.0 * mo : 0.0);
+    f = on ? f : 0.0;
+    sum.x += dx * f;
+    sum.y += dy * f;
 }
 
 // With COUNT, the kernel also counts its work for the roofline: regions
 // accepted (A) and opened (O), and node pairs (P), into cnt[0..2].
+// psort holds kLeafSize records of padding after the last node, so that the
+// nodes of a small region are loaded at once.
 template <typename R, bool ANY_ADJUST, bool COUNT = false>
 __global__ void __launch_bounds__(256)
 k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
@@ -703,7 +800,7 @@ k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
     const int k = valid ? k0 : N - 1;
     const int my_g = valid ? gid[k] : -1;
     const V4 me = psort[k];
-    Sum2<R> acc2;
+    ForceSum<R> sum;
     // the lanes of one wave may belong to several graphs of a batch: walk
     // each graph's tree in turn with its own lanes
     uint64_t todo = __ballot(valid);
@@ -716,16 +813,17 @@ k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
         const int gend = G.noff + G.n;
         const bool adjust = ANY_ADJUST && (G.flags & GF_ADJUST);
         const R theta2 = (R) G.theta2;
-        const R smi = (R) G.scaling * me.z;
-        const double sscale = G.scaling * (double) me.z; // scaling * m_i in FP64
         int done = mine ? INT_MIN + 1 : INT_MAX; // lanes of other graphs never take part
         int c = G.ioff;                          // the root (every graph has n >= 2)
         // every node of the tree is visited at most once (2n - 1 of them); the
         // bound only guards against a corrupt tree
         for (int steps = 2 * G.n; c != kEnd && steps > 0; steps--) {
             c = __builtin_amdgcn_readfirstlane(c);
+            // lane masks of the step, kept in scalar registers: the
+            // comparisons write them directly and the lanes are selected
+            // from them without per-lane selects
             int j0, j1, nxt;
-            bool take;
+            uint64_t tm; // lanes that visit the step's nodes one by one
             if (c >= 0) {
                 const TNode<R> node = tn[c]; // one scalar load of the whole record
                 const V4 rc = node.c;
@@ -735,24 +833,22 @@ k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
                                  "s"(rl.y), "s"(rl.z), "s"(rl.w));
                 const R xd = me.x - rc.x, yd = me.y - rc.y;
                 const R d2 = xd * xd + yd * yd;
-                const R smm = smi * rc.z;
-                const bool active = rl.z > done;
-                const bool far = d2 * theta2 > rc.w;
-                const bool acc = active & far;
-                take = active & !far;
-                if (COUNT) {
-                    nA += acc;
-                    nO += take;
+                const uint64_t am = __builtin_amdgcn_ballot_w64(rl.z > done);
+                const uint64_t fm = __builtin_amdgcn_ballot_w64(d2 * theta2 > rc.w);
+                if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {
+                    // the monopole, for the lanes that accept: a branch on
+                    // the lane mask (the lanes' sums are updated in place)
+                    asm volatile("");
+                    sum.add(xd, yd, rc.z * frcp_r(d2));
+                    done = rl.w;
+                    if (COUNT) nA++;
                 }
-                if (__ballot(acc)) { // the monopole, for the lanes that accept
-                    const R f = acc ? smm * frcp_r(d2) : (R) 0;
-                    acc2.add(xd * f, yd * f);
-                    done = acc ? rl.w : done;
-                }
-                if (__ballot(take) == 0) { // every lane is done with this region
+                tm = am & ~fm;
+                if (tm == 0) { // every lane is done with this region
                     c = rl.y;
                     continue;
                 }
+                if (COUNT) nO += __builtin_amdgcn_inverse_ballot_w64(tm);
                 if (rl.w - rl.z >= kLeafSize) { // descend
                     c = rl.x;
                     continue;
@@ -762,27 +858,47 @@ k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
                 nxt = rl.y;
             } else {
                 j0 = j1 = ~c;
-                take = j0 > done;
+                tm = __builtin_amdgcn_ballot_w64(j0 > done);
                 nxt = j0 == gend - 1 ? kEnd : escape[j0 - g];
             }
-            for (int j = j0; j <= j1; j++) {
-                const V4 o = psort[j];
-                const bool pair = take & (j != k);
-                if (adjust)
-                    lane_pair<R, true>(me, o, smi, sscale, pair, acc2);
-                else
-                    lane_pair<R, false>(me, o, smi, sscale, pair, acc2);
-                if (COUNT) nP += pair;
+            // up to kLeafSize nodes, loaded together (psort is padded)
+            V4 o[kLeafSize];
+#pragma unroll
+            for (int u = 0; u < kLeafSize; u++) o[u] = psort[j0 + u];
+            if (sizeof(R) == 4) // issued together, one wait for all of them
+                asm volatile("" ::"s"(o[0].x), "s"(o[0].y), "s"(o[0].z), "s"(o[1].x), "s"(o[1].y),
+                             "s"(o[1].z), "s"(o[2].x), "s"(o[2].y), "s"(o[2].z), "s"(o[3].x),
+                             "s"(o[3].y), "s"(o[3].z), "s"(o[4].x), "s"(o[4].y), "s"(o[4].z),
+                             "s"(o[5].x), "s"(o[5].y), "s"(o[5].z), "s"(o[6].x), "s"(o[6].y),
+                             "s"(o[6].z), "s"(o[7].x), "s"(o[7].y), "s"(o[7].z));
+            if (__builtin_amdgcn_inverse_ballot_w64(tm)) {
+                // a lane meets itself with xd = yd = 0, which adds nothing:
+                // d2 = 0 gives a zero weight, and with sizes an overlap
+                // weight times a zero offset
+#pragma unroll
+                for (int u = 0; u < kLeafSize; u++) {
+                    if (j0 + u > j1) break;
+                    if (COUNT) nP += (j0 + u) != k;
+                    if (adjust) {
+                        lane_pair_adjust<R>(me, o[u], sum, true);
+                    } else {
+                        const R xd = me.x - o[u].x, yd = me.y - o[u].y;
+                        const R d2 = xd * xd + yd * yd;
+                        const R w = d2 > (R) 0 ? o[u].z * frcp_r(d2) : (R) 0;
+                        sum.add(xd, yd, w);
+                    }
+                }
+                done = j1;
             }
-            done = take ? j1 : done;
             c = nxt;
         }
     }
     if (valid) {
         // gravity (pipeline.cl:404-411): the force leaves this kernel as
         // repulsion minus gravity, the attraction kernels add the edges
-        double fx = acc2.x, fy = acc2.y;
         const GraphDesc G = gd[my_g];
+        const double sscale = G.scaling * (double) me.z; // scaling * m_i in FP64
+        double fx = sscale * sum.x, fy = sscale * sum.y;
         const R grav = (R) G.gravity;
         if (grav != (R) 0) {
             const R dist = fsqrt_r(me.x * me.x + me.y * me.y);
@@ -838,15 +954,14 @@ __device__ inline void attract_term(typename Vec<R>::v2 pi, typename Vec<R>::v2 
 }
 
 // The incidences e0, e0 + step, ... < e1 of one node, in that order; the
-// loads of four incidences are issued before their arithmetic.
-template <typename R>
+// loads of U incidences are issued before their arithmetic.
+template <typename R, int U>
 __device__ inline void attract_terms(int e0, int e1, int step, bool linlog, bool adjust,
                                      typename Vec<R>::v2 pi, R si,
                                      const typename Vec<R>::v2 *__restrict__ pos,
                                      const R *__restrict__ size, const int *__restrict__ inc_node,
                                      const R *__restrict__ inc_coef, KahanSum2<R> &sum)
 {
-    constexpr int U = 4;
     for (int e = e0; e < e1; e += U * step) {
         int j[U];
         R c[U];
@@ -884,7 +999,9 @@ k_attract(int count, const int *__restrict__ nodes, const int *__restrict__ gid,
     const int flags = gd[gid[i]].flags;
     KahanSum2<R> sum;
     if (live)
-        attract_terms<R>(inc_off[i] + sub, inc_off[i + 1], GL, flags & GF_LINLOG,
+        // a lane per node: four incidences in flight per lane; 16 lanes per
+        // node already keep enough loads in flight
+        attract_terms<R, GL == 1 ? 4 : 1>(inc_off[i] + sub, inc_off[i + 1], GL, flags & GF_LINLOG,
                          flags & GF_ADJUST, pos[i], size[i], pos, size, inc_node, inc_coef, sum);
     R ax = sum.sum_x(), ay = sum.sum_y();
 #pragma unroll
@@ -915,7 +1032,7 @@ k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict
     const int i = it.x;
     const int flags = gd[gid[i]].flags;
     KahanSum2<R> sum;
-    attract_terms<R>(it.y + lane, it.z, 64, flags & GF_LINLOG, flags & GF_ADJUST, pos[i], size[i],
+    attract_terms<R, 1>(it.y + lane, it.z, 64, flags & GF_LINLOG, flags & GF_ADJUST, pos[i], size[i],
                      pos, size, inc_node, inc_coef, sum);
     R ax = sum.sum_x(), ay = sum.sum_y();
 #pragma unroll
@@ -1199,6 +1316,7 @@ template <typename R> struct EngineImpl final : Engine {
     int speed_state(int graph, double out[4]) override;
     int tree(int graph, uint32_t *keys_sorted, uint32_t *order, int *child, int *range,
              int *parent) override;
+    int regions(int graph, double *rec, int *skip) override;
     int stage_times(double *t) override;
     int graph_status(int graph) override;
     int count_work(unsigned long long out[3]) override;
@@ -1358,7 +1476,7 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     hs.check(d_hubpart.alloc(nslots), "alloc");
     hs.check(d_tbox.alloc(T), "alloc");
     hs.check(d_gbox.alloc(G), "alloc");
-    hs.check(d_psort.alloc(nn), "alloc");
+    hs.check(d_psort.alloc(nn + kLeafSize), "alloc");
     hs.check(d_nd.alloc(2 * (size_t) (NI > 0 ? NI : 1)), "alloc");
     hs.check(d_keyA.alloc(nn), "alloc");
     hs.check(d_valA.alloc(nn), "alloc");
@@ -1406,6 +1524,8 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     for (int c = 0; c < 2; c++) up(d_class[c].p, h_class[c].data(), h_class[c].size() * sizeof(int));
     up(d_items.p, h_items.data(), h_items.size() * sizeof(int4));
     up(d_hubs.p, h_hubs.data(), h_hubs.size() * sizeof(int4));
+    if (hs.ok())
+        hs.check(hipMemsetAsync(d_psort.p + nn, 0, kLeafSize * sizeof(V4), stream), "memset");
     if (hs.ok()) hs.check(hipMemsetAsync(d_F[0].p, 0, nn * sizeof(double2), stream), "memset");
     if (hs.ok()) hs.check(hipMemsetAsync(d_F[1].p, 0, nn * sizeof(double2), stream), "memset");
     if (hs.ok()) {
@@ -1467,7 +1587,7 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
         k_karras<R><<<ceil_div(NI, 256), 256, 0, stream>>>(
             NI, d_gi.p, d_gd.p, d_keyA.p, d_child.p, d_tn.p, d_parent_leaf.p, d_parent_int.p,
             d_escape.p, d_visits.p, NS, d_rcnt.p);
-        k_aggregate_tile<R><<<T, 1024, 0, stream>>>(d_tiles.p, d_gd.p, d_psort.p, d_child.p,
+        k_aggregate_tile<R><<<T * (kTile / agg_block<R>()), 1024, 0, stream>>>(d_tiles.p, d_gd.p, d_psort.p, d_child.p,
                                                      d_parent_leaf.p, d_parent_int.p, d_escape.p,
                                                      NS, N, d_visits.p, d_ready.p, d_rcnt.p,
                                                      d_nd.p, d_tn.p);
@@ -1719,6 +1839,39 @@ int EngineImpl<R>::tree(int graph, uint32_t *keys_sorted, uint32_t *order, int *
     }
     if (parent)
         for (int k = 0; k < n; k++) parent[n - 1 + k] = pl[k] - D.ioff;
+    return 0;
+}
+
+template <typename R> int EngineImpl<R>::regions(int graph, double *rec, int *skip)
+{
+    if (graph < 0 || graph >= G) {
+        set_message("no graph %d in this batch", graph);
+        return 1;
+    }
+    (void) hipSetDevice(device);
+    const GraphDesc &D = h_gd[graph];
+    const int n = D.n;
+    std::vector<TNode<R>> tn(n - 1);
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess)
+        e = hipMemcpy(tn.data(), d_tn.p + D.ioff, (n - 1) * sizeof(TNode<R>),
+                      hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        set_hip_error("could not fetch the tree from the device", e);
+        return 1;
+    }
+    for (int r = 0; r < n - 1; r++) {
+        if (rec) {
+            rec[4 * r] = (double) tn[r].c.x;
+            rec[4 * r + 1] = (double) tn[r].c.y;
+            rec[4 * r + 2] = (double) tn[r].c.z;
+            rec[4 * r + 3] = (double) tn[r].c.w;
+        }
+        if (skip) {
+            const int s = tn[r].l.y;
+            skip[r] = s == kEnd ? -1 : s >= 0 ? s - D.ioff : n - 1 + (~s - D.noff);
+        }
+    }
     return 0;
 }
 

@@ -84,6 +84,14 @@ int fa2_batch_speed(fa2_batch *b, int g, double out[4]);
 int fa2_batch_tree(fa2_batch *b, int g, uint32_t *keys_sorted, uint32_t *order, int *child,
                    int *range, int *parent);
 
+/* The traversal records of the regions of graph g at its last iteration, in
+ * the numbering of the reference: rec (4(n-1)) = (centre x, centre y, mass,
+ * size^2) per region (pipeline.cl:312-323); skip (n-1) = the region (0..n-2)
+ * or leaf (n-1..2n-2) that the traversal visits after the region's subtree,
+ * -1 after the last. Either pointer may be NULL. The reference keeps these
+ * records inside the OpenCL buffers and has no entry point for them. */
+int fa2_batch_regions(fa2_batch *b, int g, double *rec, int *skip);
+
 /* Device time of each phase (FA2_FULL_NTIMES slots, seconds, slot 0 unused)
  * summed over the iterations run with profile set. */
 int fa2_batch_stage_times(fa2_batch *b, double *times);

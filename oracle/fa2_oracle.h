@@ -53,6 +53,8 @@ int orc_large_prepare(int n, int m, const int *from, const int *to,
     void orc_large_tree_##S(int n, const PR *x, const PR *y, uint32_t *key,   \
                             uint32_t *key_sorted, uint32_t *order,            \
                             int *child, int *range, int *parent);             \
+    int orc_large_regions_##S(int n, const PR *x, const PR *y,                \
+                              const double *mass, double *ni);                \
     int orc_large_run_##S(const orc_large_params *p, double *pos,             \
                           int iterations, double *trace_pos,                  \
                           double *trace_force, double *trace_speed,           \

@@ -93,6 +93,65 @@ static void SFX(agg_node)(int n, int r, const int *child, FR *nd)
     }
 }
 
+/* The traversal records of the regions (pipeline.cl:276-323) on the tree of
+ * the positions: ni[4r..4r+3] = (centre x, centre y, mass, size^2) of region
+ * r in the reference numbering; mass[i] is the mass of node i. */
+int SFX(orc_large_regions)(int n, const PR *x, const PR *y, const double *mass,
+                           double *ni_out)
+{
+    if (n < 2) return 1;
+    const size_t nn = (size_t) n;
+    uint32_t *key = (uint32_t *) malloc(nn * sizeof(uint32_t));
+    uint32_t *ks = (uint32_t *) malloc(nn * sizeof(uint32_t));
+    uint32_t *order = (uint32_t *) malloc(nn * sizeof(uint32_t));
+    int *child = (int *) malloc(2 * (nn - 1) * sizeof(int));
+    int *range = (int *) malloc(2 * (nn - 1) * sizeof(int));
+    int *parent = (int *) malloc((2 * nn - 1) * sizeof(int));
+    FR *nd = (FR *) malloc(8 * (2 * nn - 1) * sizeof(FR));
+    int status = 1;
+    if (!key || !ks || !order || !child || !range || !parent || !nd) goto out;
+    SFX(orc_large_tree)(n, x, y, key, ks, order, child, range, parent);
+    for (int k = 0; k < n; k++) {
+        const uint32_t i = order[k];
+        FR *L = nd + 8 * (nn - 1 + (size_t) k);
+        const FR px = POS_FR(x[i]), py = POS_FR(y[i]), pm = (FR) mass[i];
+        L[0] = px;
+        L[1] = py;
+        L[2] = px;
+        L[3] = py;
+        L[4] = pm;
+        L[5] = px * pm;
+        L[6] = py * pm;
+        L[7] = (FR) 1;
+    }
+    for (int r = 0; r < n - 1; r++) nd[8 * r + 7] = (FR) 0;
+    SFX(agg_node)(n, 0, child, nd);
+    for (int r = 0; r < n - 1; r++) {
+        const FR m = nd[8 * r + 4];
+        const FR w = nd[8 * r + 2] - nd[8 * r];
+        const FR h = nd[8 * r + 3] - nd[8 * r + 1];
+#ifdef ORC_CLAMP_CENTRE
+        ni_out[4 * r] = (double) ORC_MIN(ORC_MAX(nd[8 * r + 5] / m, nd[8 * r]), nd[8 * r + 2]);
+        ni_out[4 * r + 1] = (double) ORC_MIN(ORC_MAX(nd[8 * r + 6] / m, nd[8 * r + 1]), nd[8 * r + 3]);
+#else
+        ni_out[4 * r] = (double) (nd[8 * r + 5] / m);
+        ni_out[4 * r + 1] = (double) (nd[8 * r + 6] / m);
+#endif
+        ni_out[4 * r + 2] = (double) m;
+        ni_out[4 * r + 3] = (double) ((FR) 2 * (w * w + h * h));
+    }
+    status = 0;
+out:
+    free(key);
+    free(ks);
+    free(order);
+    free(child);
+    free(range);
+    free(parent);
+    free(nd);
+    return status;
+}
+
 static inline FR SFX(fsqrt)(FR v) { return sizeof(FR) == 4 ? (FR) sqrtf((float) v) : (FR) sqrt((double) v); }
 static inline FR SFX(flog1p)(FR v) { return sizeof(FR) == 4 ? (FR) log1pf((float) v) : (FR) log1p((double) v); }
 

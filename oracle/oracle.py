@@ -50,6 +50,7 @@ def _load():
     for s in "fdm":
         protos[f"orc_morton_{s}"] = (None, [c_int, V, V, V])
         protos[f"orc_large_tree_{s}"] = (None, [c_int, V, V, V, V, V, V, V, V])
+        protos[f"orc_large_regions_{s}"] = (c_int, [c_int, V, V, V, V])
         protos[f"orc_large_run_{s}"] = (c_int, [POINTER(LargeParams), V, c_int,
                                                 V, V, V, V])
     for name, (res, args) in protos.items():
@@ -193,6 +194,21 @@ def large_tree(x, y, mode="f"):
                                            _p(parent))
     return dict(keys_unsorted=key, keys=ks, order=order, child=child,
                 range=rng_, parent=parent)
+
+
+def large_regions(x, y, mass, mode="f"):
+    """Traversal records of the regions of the tree of (x, y): an (n-1, 4)
+    array of (centre x, centre y, mass, size^2) in the reference numbering
+    (pipeline.cl:276-323); mode as for large_run."""
+    n = len(x)
+    dt = np.float32 if mode == "f" else np.float64
+    x = np.ascontiguousarray(x, dt)
+    y = np.ascontiguousarray(y, dt)
+    m = np.ascontiguousarray(mass, np.float64)
+    ni = np.zeros(4 * (n - 1))
+    if getattr(lib, f"orc_large_regions_{mode}")(n, _p(x), _p(y), _p(m), _p(ni)) != 0:
+        raise ValueError("orc_large_regions failed")
+    return ni.reshape(n - 1, 4)
 
 
 def large_prepare(n, edges, w=None, outbound=False):

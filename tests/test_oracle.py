@@ -351,3 +351,38 @@ def test_large_prepare_matches_product_preprocessing():
         b = prepare_large(n, e[:, 0], e[:, 1], w, outbound)
         for x, y in zip(a, b):
             assert np.array_equal(np.asarray(x), np.asarray(y))
+
+
+def test_large_regions_equal_sums_over_each_region():
+    """Region records (pipeline.cl:276-323) against sums taken directly over
+    the nodes each region covers in the oracle's own tree: mass, weighted
+    centre, size^2 = 2 (w^2 + h^2) of the bounding box; coincident nodes keep
+    the centre inside the box (modes d and m)."""
+    rng = np.random.default_rng(5)
+    n = 1500
+    x = rng.uniform(-300, 300, n)
+    y = rng.uniform(-80, 80, n)
+    x[:40] = x[np.arange(40) % 4]
+    y[:40] = y[np.arange(40) % 4]
+    m = rng.integers(1, 30, n).astype(np.float64)
+    for mode in ("d", "m"):
+        px = x.astype(np.float32).astype(np.float64) if mode == "m" else x
+        py = y.astype(np.float32).astype(np.float64) if mode == "m" else y
+        rec = O.large_regions(x, y, m, mode)
+        t = O.large_tree(x, y, mode if mode == "d" else "f")
+        order = t["order"].astype(np.int64)
+        rg = t["range"].reshape(-1, 2)
+        for r in range(n - 1):
+            idx = order[rg[r, 0]:rg[r, 1] + 1]
+            M = m[idx].sum()
+            cx = (m[idx] * px[idx]).sum() / M
+            cy = (m[idx] * py[idx]).sum() / M
+            w = px[idx].max() - px[idx].min()
+            h = py[idx].max() - py[idx].min()
+            assert rec[r, 2] == M
+            scale = max(np.abs(px[idx]).max(), np.abs(py[idx]).max())
+            assert abs(rec[r, 0] - cx) <= 1e-12 * scale
+            assert abs(rec[r, 1] - cy) <= 1e-12 * scale
+            assert px[idx].min() <= rec[r, 0] <= px[idx].max()
+            assert py[idx].min() <= rec[r, 1] <= py[idx].max()
+            assert rec[r, 3] == pytest.approx(2 * (w * w + h * h), rel=1e-12, abs=1e-9)
