@@ -13,7 +13,17 @@ Printed: one JSON line with node-iters/s, the roofline of the dominant
 kernel (timed with HIP events on the batch's stream over the timed region)
 and the CPU baseline (the C restatement of the reference CPU layout,
 Gephi Barnes-Hut in FP64 with OpenMP, on a bounded sample of iterations of
-the same graph).
+the same graph). Beside it, in the same line:
+  * "batched": 512 stochastic-block-model graphs (200..20000 nodes,
+    log-uniform; 2..20 equal blocks; mean in-block degree 8, cross-block 1),
+    200 iterations each, laid out as one batch and one at a time;
+  * "serving": a Poisson stream of layout requests (preferential-attachment
+    or block-model graphs of 1k..1M nodes, log-uniform; mean degree 4..16;
+    100..1000 iterations; linlog, prevent_overlap with node sizes in [1, 3],
+    5% pinned nodes and double precision mixed in) served by the layout
+    server, against the same requests solved one at a time; p50/p99
+    latency and layouts/s. Both start from prepared incidence lists and
+    include the upload and the download of every graph.
 """
 import argparse
 import json
@@ -70,6 +80,159 @@ def cpu_baseline(n, e, pos, args):
                         f"{threads} OpenMP threads, {dt:.1f} s"))
 
 
+def batched_leg(args):
+    """512 SBM graphs, 200 iterations: one batch against one at a time."""
+    from forceatlas2_rc_amd import Batch, graphs
+    from forceatlas2_rc_amd.api import prepare_large
+    rng = np.random.default_rng(args.seed + 4)
+    gs, n_tot, m_tot = [], 0, 0
+    t0 = time.time()
+    for g in range(args.batch_graphs):
+        n = int(round(np.exp(rng.uniform(np.log(200), np.log(20000)))))
+        k = int(rng.integers(2, 21))
+        n, e = graphs.sbm(n, k, deg_in=8.0, deg_out=1.0, seed=args.seed + 1000 + g)
+        r = 10.0 * np.sqrt(n)
+        pos = rng.uniform(-r, r, size=(n, 2))
+        mass, off, node, coef = prepare_large(n, e[:, 0], e[:, 1], np.ones(len(e)), False)
+        gs.append(dict(n=n, frm=e[:, 0], to=e[:, 1], mass=mass, inc_off=off, inc_node=node,
+                       inc_coef=coef, init=pos, theta=1.2, iterations=args.batch_iters))
+        n_tot += n
+        m_tot += len(e)
+    log(f"batched: {len(gs)} graphs, {n_tot} nodes, {m_tot} edges ({time.time() - t0:.1f} s)")
+    it = args.batch_iters
+    # warm-up: code objects, allocator
+    with Batch(gs[:2]) as b:
+        b.run(2)
+        b.sync()
+    t0 = time.perf_counter()
+    b = Batch(gs)
+    t_up = time.perf_counter()
+    b.run(it)
+    b.sync()
+    t_run = time.perf_counter()
+    res_b = [b.positions(g) for g in range(len(gs))]
+    b.close()
+    t_batch = time.perf_counter() - t0
+    log(f"batched: {t_batch:.2f} s ({(t_run - t_up) / it * 1e3:.3f} ms per iteration)")
+    t0 = time.perf_counter()
+    same = 0
+    for g, gr in enumerate(gs):
+        with Batch([gr]) as b1:
+            b1.run(it)
+            b1.sync()
+            same += int(np.array_equal(b1.positions(0), res_b[g]))
+    t_one = time.perf_counter() - t0
+    log(f"one at a time: {t_one:.2f} s")
+    return dict(workload=(f"{len(gs)} SBM graphs, 200..20000 nodes log-uniform, 2..20 blocks, "
+                          f"in-block degree 8, cross-block 1, {it} iterations each, BH theta=1.2"),
+                graphs=len(gs), nodes=n_tot, edges=m_tot, iterations=it,
+                batched_s=t_batch, batched_ms_per_iteration=(t_run - t_up) / it * 1e3,
+                one_at_a_time_s=t_one, layouts_per_s_batched=len(gs) / t_batch,
+                layouts_per_s_one_at_a_time=len(gs) / t_one, speedup=t_one / t_batch,
+                node_iters_per_s_batched=n_tot * it / t_batch,
+                identical_results=f"{same}/{len(gs)}")
+
+
+def serving_requests(args):
+    from forceatlas2_rc_amd import graphs
+    from forceatlas2_rc_amd.api import prepare_large
+    rng = np.random.default_rng(args.seed + 5)
+    reqs = []
+    for q in range(args.serve_requests):
+        n = int(round(np.exp(rng.uniform(np.log(1000), np.log(args.serve_max_nodes)))))
+        deg = rng.uniform(4.0, 16.0)
+        if rng.random() < 0.5:
+            n, e = graphs.preferential_attachment(n, max(1, int(round(deg / 2))),
+                                                  seed=args.seed + 2000 + q)
+            kind = "pa"
+        else:
+            n, e = graphs.sbm(n, int(rng.integers(2, 21)), deg_in=deg - 1.0, deg_out=1.0,
+                              seed=args.seed + 2000 + q)
+            kind = "sbm"
+        r = 10.0 * np.sqrt(n)
+        pos = rng.uniform(-r, r, size=(n, 2))
+        g = dict(n=n, frm=e[:, 0], to=e[:, 1], init=pos, theta=1.2,
+                 iterations=int(rng.integers(100, 1001)))
+        if rng.random() < 0.5:
+            g["linlog"] = True
+        if rng.random() < 0.3:
+            g["adjust"] = True
+            g["size"] = rng.uniform(1.0, 3.0, n)
+        if rng.random() < 0.5:
+            g["fixed"] = rng.random(n) < 0.05
+        outbound = rng.random() < 0.3
+        g["dissuade_hubs"] = outbound
+        mass, off, node, coef = prepare_large(n, e[:, 0], e[:, 1], np.ones(len(e)), outbound)
+        g.update(mass=mass, inc_off=off, inc_node=node, inc_coef=coef)
+        prec = "double" if rng.random() < 0.25 else "single"
+        reqs.append((g, prec, kind))
+    gaps = rng.exponential(1.0 / args.serve_rate, len(reqs))
+    gaps[0] = 0.0
+    return reqs, np.cumsum(gaps)
+
+
+def serving_leg(args):
+    """A Poisson stream of layout requests through the server, against the
+    same requests solved one at a time."""
+    from forceatlas2_rc_amd import Batch, Server
+    t0 = time.time()
+    reqs, arrive = serving_requests(args)
+    n_tot = sum(g["n"] for g, _, _ in reqs)
+    work = sum(g["n"] * g["iterations"] for g, _, _ in reqs)
+    log(f"serving: {len(reqs)} requests, {n_tot} nodes, {work:.3g} node-iterations "
+        f"({time.time() - t0:.1f} s)")
+    with Batch([reqs[0][0]]) as b:  # warm-up
+        b.run(2)
+        b.sync()
+    with Server(max_batches=args.serve_batches, chunk=args.serve_chunk) as srv:
+        ids = []
+        t0 = time.perf_counter()
+        for (g, prec, _), ta in zip(reqs, arrive):
+            wait = t0 + ta - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            ids.append(srv.submit(g, prec))
+        out = [srv.result(j) for j in ids]
+        t_end = time.perf_counter()
+        stats = srv.stats()
+    lat = np.array([t["total"] for _, t in out])
+    span = t_end - t0
+    log(f"served: {span:.2f} s, p50 {np.percentile(lat, 50):.2f} s, "
+        f"p99 {np.percentile(lat, 99):.2f} s, {stats}")
+    # one at a time, in arrival order
+    svc, same = [], 0
+    for k, (g, prec, _) in enumerate(reqs):
+        t1 = time.perf_counter()
+        with Batch([g], precision=prec) as b:
+            b.run(g["iterations"])
+            b.sync()
+            p = b.positions(0)
+        svc.append(time.perf_counter() - t1)
+        same += int(np.array_equal(p, out[k][0]))
+    svc = np.array(svc)
+    # the same arrivals served by one solver in turn
+    fin, lat1 = 0.0, []
+    for ta, s_ in zip(arrive, svc):
+        fin = max(fin, ta) + s_
+        lat1.append(fin - ta)
+    lat1 = np.array(lat1)
+    log(f"one at a time: {svc.sum():.2f} s of service")
+    return dict(workload=(f"Poisson stream, {args.serve_rate:g} requests/s, {len(reqs)} "
+                          f"requests; preferential-attachment or SBM graphs of 1000.."
+                          f"{args.serve_max_nodes} nodes (log-uniform), mean degree 4..16, "
+                          "100..1000 iterations; linlog, prevent_overlap (sizes 1..3), 5% "
+                          "pinned, dissuade_hubs and double precision mixed in"),
+                requests=len(reqs), nodes=n_tot, node_iterations=work,
+                layouts_per_s=len(reqs) / span, p50_s=float(np.percentile(lat, 50)),
+                p99_s=float(np.percentile(lat, 99)),
+                one_at_a_time=dict(layouts_per_s=len(reqs) / svc.sum(),
+                                   p50_s=float(np.percentile(lat1, 50)),
+                                   p99_s=float(np.percentile(lat1, 99))),
+                speedup=(len(reqs) / span) / (len(reqs) / svc.sum()),
+                batches=stats["batches"], max_batches_running=stats["peak_running"],
+                identical_results=f"{same}/{len(reqs)}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +244,15 @@ def main():
     ap.add_argument("--precision", default="single")
     ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-graphs", type=int, default=512)
+    ap.add_argument("--batch-iters", type=int, default=200)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-max-nodes", type=int, default=1_000_000)
+    ap.add_argument("--serve-batches", type=int, default=4)
+    ap.add_argument("--serve-chunk", type=int, default=25)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many "
                          "bytes (a known-size reference for HBM counters)")
@@ -167,6 +339,11 @@ def main():
         log("cpu baseline...")
         out["cpu_baseline"] = cpu_baseline(n, e, pos, args)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    del graph, mass, off, node, coef
+    if not args.no_batched:
+        out["batched"] = batched_leg(args)
+    if not args.no_serving:
+        out["serving"] = serving_leg(args)
     print(json.dumps(out), flush=True)
 
 

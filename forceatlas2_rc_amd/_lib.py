@@ -106,6 +106,13 @@ PROTOTYPES = {
                                     c_double, c_double]),
     "fa2_batch_stream": (c_void_p, [c_void_p]),
     "fa2_batch_close": (None, [c_void_p]),
+    "fa2_serve_open": (c_void_p, [c_int, c_double, c_int, c_int]),
+    "fa2_serve_submit": (c_int, [c_void_p, POINTER(FullParams), c_void_p]),
+    "fa2_serve_status": (c_int, [c_void_p, c_int]),
+    "fa2_serve_wait": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "fa2_serve_stats": (c_int, [c_void_p, c_void_p]),
+    "fa2_serve_close": (None, [c_void_p]),
+    "fa2_serve_footprint": (c_double, [c_int, c_int, ctypes.c_longlong]),
 }
 
 

@@ -284,6 +284,54 @@ def prepare_large(n, frm, to, w, outbound):
     return mass, inc_off, inc_node, inc_coef
 
 
+def _fill_params(p, gr, device, precision, profile):
+    """Fill the fa2_full_params p from the graph dict gr (see Batch); returns
+    the arrays p points into (to be kept alive) and the positions (x then
+    y)."""
+    n = int(gr["n"])
+    if "frm" in gr:
+        frm = np.asarray(gr["frm"], np.int32)
+        to = np.asarray(gr["to"], np.int32)
+    else:
+        frm, to = _edges(n, gr.get("edges"))
+    w = np.ones(frm.size) if gr.get("weights") is None \
+        else np.asarray(gr["weights"], np.float64)
+    outbound = bool(gr.get("dissuade_hubs", False))
+    if "inc_off" in gr:
+        mass, off, node, coef = (gr["mass"], gr["inc_off"], gr["inc_node"], gr["inc_coef"])
+    else:
+        mass, off, node, coef = prepare_large(n, frm, to, w, outbound)
+    adjust = bool(gr.get("adjust", False))
+    size = np.ascontiguousarray(
+        np.broadcast_to(np.asarray(gr.get("size", 1.0), np.float64), (n,)))
+    fixed = gr.get("fixed")
+    fixed = np.zeros(n, np.int32) if fixed is None else np.asarray(fixed, np.int32)
+    init = np.asarray(gr["init"], np.float64)
+    xy = np.ascontiguousarray(init.T).reshape(-1).copy()
+    p.n = n
+    p.inc_off = off.ctypes.data_as(POINTER(c_int))
+    p.inc_node = node.ctypes.data_as(POINTER(c_int))
+    p.inc_coef = coef.ctypes.data_as(POINTER(c_double))
+    p.mass = mass.ctypes.data_as(POINTER(c_double))
+    p.size = size.ctypes.data_as(POINTER(c_double))
+    p.fixed = fixed.ctypes.data_as(POINTER(c_int))
+    p.any_fixed = int(fixed.any())
+    p.scaling = float(gr.get("scaling", 2.0 if n >= 100 else 10.0))
+    p.gravity = float(gr.get("gravity", 1.0))
+    p.theta = float(gr.get("theta", 1.2))
+    p.jitter_tolerance = float(gr.get("jitter_tolerance", 1.0))
+    p.strong = int(bool(gr.get("strong", False)))
+    p.linlog = int(bool(gr.get("linlog", False)))
+    p.adjust = int(adjust)
+    p.iterations = int(gr.get("iterations", 1 << 30))
+    p.device = int(device)
+    p.use_double = int(precision == "double")
+    p.profile = int(bool(profile))
+    p.source = None
+    p.interrupted = INTERRUPT_FN(0)
+    return [mass, off, node, coef, size, fixed, xy], xy
+
+
 class Batch:
     """Graphs laid out together on one device (fa2_batch_* of fa2_hip.h).
 
@@ -301,53 +349,10 @@ class Batch:
         pos_ptrs = (c_void_p * G)()
         self.n = []
         for g, gr in enumerate(graphs):
-            n = int(gr["n"])
-            if "frm" in gr:
-                frm = np.asarray(gr["frm"], np.int32)
-                to = np.asarray(gr["to"], np.int32)
-            else:
-                frm, to = _edges(n, gr.get("edges"))
-            w = np.ones(frm.size) if gr.get("weights") is None \
-                else np.asarray(gr["weights"], np.float64)
-            outbound = bool(gr.get("dissuade_hubs", False))
-            if "inc_off" in gr:
-                mass, off, node, coef = (gr["mass"], gr["inc_off"],
-                                         gr["inc_node"], gr["inc_coef"])
-            else:
-                mass, off, node, coef = prepare_large(n, frm, to, w, outbound)
-            adjust = bool(gr.get("adjust", False))
-            size = np.ascontiguousarray(
-                np.broadcast_to(np.asarray(gr.get("size", 1.0), np.float64), (n,)))
-            fixed = gr.get("fixed")
-            fixed = np.zeros(n, np.int32) if fixed is None \
-                else np.asarray(fixed, np.int32)
-            init = np.asarray(gr["init"], np.float64)
-            xy = np.ascontiguousarray(init.T).reshape(-1).copy()
-            p = params[g]
-            p.n = n
-            p.inc_off = off.ctypes.data_as(POINTER(c_int))
-            p.inc_node = node.ctypes.data_as(POINTER(c_int))
-            p.inc_coef = coef.ctypes.data_as(POINTER(c_double))
-            p.mass = mass.ctypes.data_as(POINTER(c_double))
-            p.size = size.ctypes.data_as(POINTER(c_double))
-            p.fixed = fixed.ctypes.data_as(POINTER(c_int))
-            p.any_fixed = int(fixed.any())
-            p.scaling = float(gr.get("scaling", 2.0 if n >= 100 else 10.0))
-            p.gravity = float(gr.get("gravity", 1.0))
-            p.theta = float(gr.get("theta", 1.2))
-            p.jitter_tolerance = float(gr.get("jitter_tolerance", 1.0))
-            p.strong = int(bool(gr.get("strong", False)))
-            p.linlog = int(bool(gr.get("linlog", False)))
-            p.adjust = int(adjust)
-            p.iterations = int(gr.get("iterations", 1 << 30))
-            p.device = int(device)
-            p.use_double = int(precision == "double")
-            p.profile = int(bool(profile))
-            p.source = None
-            p.interrupted = INTERRUPT_FN(0)
+            keep, xy = _fill_params(params[g], gr, device, precision, profile)
             pos_ptrs[g] = xy.ctypes.data
-            self._keep += [mass, off, node, coef, size, fixed, xy]
-            self.n.append(n)
+            self._keep += keep
+            self.n.append(int(gr["n"]))
         self.h = lib.fa2_batch_open(G, params, pos_ptrs)
         self._keep = None
         if not self.h:
@@ -448,3 +453,74 @@ class Batch:
 
     def __exit__(self, *a):
         self.close()
+
+
+class Server:
+    """A layout server on one device (fa2_serve_* of fa2_hip.h): requests
+    are admitted by device-memory footprint into batches that run at the
+    same time on their own streams; each request's positions come back as
+    soon as its own iterations are done.
+
+    ``submit(graph, precision)`` takes a graph dict as for Batch (with
+    ``iterations``) and returns a request id; ``result(id)`` waits for it
+    and returns the positions (n, 2) and the times (queued, running, total)
+    in seconds. Requests of both precisions can be mixed: each batch holds
+    one precision.
+    """
+
+    def __init__(self, device=0, budget_bytes=0.0, max_batches=4, chunk=25):
+        self.h = lib.fa2_serve_open(int(device), float(budget_bytes), int(max_batches),
+                                    int(chunk))
+        if not self.h:
+            raise GPUError(last_error())
+        self.device = int(device)
+        self.n = {}
+
+    def submit(self, graph, precision="single"):
+        if precision not in ("single", "double"):
+            raise ValueError("precision must be 'single' or 'double'")
+        p = FullParams()
+        keep, xy = _fill_params(p, graph, self.device, precision, False)
+        job = lib.fa2_serve_submit(self.h, ctypes.byref(p), xy.ctypes.data)
+        del keep
+        if job < 0:
+            raise GPUError(last_error())
+        self.n[job] = int(graph["n"])
+        return job
+
+    def status(self, job):
+        return ("queued", "running", "done", "failed")[lib.fa2_serve_status(self.h, int(job))]
+
+    def result(self, job):
+        n = self.n[job]
+        xy = np.empty(2 * n)
+        t = np.zeros(3)
+        if lib.fa2_serve_wait(self.h, int(job), _ptr(xy), _ptr(t)) != 0:
+            raise GPUError(last_error())
+        return xy.reshape(2, n).T.copy(), dict(queued=t[0], running=t[1], total=t[2])
+
+    def stats(self):
+        out = np.zeros(4)
+        lib.fa2_serve_stats(self.h, _ptr(out))
+        return dict(batches=int(out[0]), admitted=int(out[1]), peak_running=int(out[2]),
+                    budget_bytes=out[3])
+
+    def close(self):
+        if self.h:
+            lib.fa2_serve_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+
+def serve_footprint(n, incidences, precision="single"):
+    """Device bytes a request of n nodes and `incidences` incidence entries
+    (twice the edges without self-loops) takes on the server."""
+    return lib.fa2_serve_footprint(int(precision == "double"), int(n), int(incidences))

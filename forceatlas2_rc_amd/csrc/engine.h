@@ -22,6 +22,10 @@ struct Engine {
     hipStream_t stream = nullptr;
     int profile = 0;
     int (*interrupted)(void) = nullptr;
+    // allocate and free the device memory in the order of `stream`
+    int async_alloc = 0;
+    // replay the iterations from a captured HIP graph (without profile)
+    int use_graph = 1;
     virtual ~Engine() = default;
     // Upload the graphs; 0 on success (message in fa2_gpu_error()).
     virtual int open(const EngineGraph *graphs, int ngraphs) = 0;
@@ -40,6 +44,14 @@ struct Engine {
     // end). Either pointer may be NULL.
     virtual int regions(int graph, double *rec, int *skip) = 0;
     virtual int stage_times(double *t) = 0;
+    // Stop computing graph `graph` (its positions and forces stay as they
+    // are): the batch keeps its slot, but its repulsion and attraction are
+    // skipped from the next iteration on.
+    virtual int retire(int graph) = 0;
+    // Iterations applied to each graph so far and its status (0, or 2 once
+    // it produced invalid numbers: it then stops), ngraphs values each, read
+    // from the device after the work enqueued before the call.
+    virtual int progress(int *done, int *status) = 0;
     virtual int graph_status(int graph) = 0;
     // Work of the repulsion on the current tree: regions accepted, regions
     // opened, node pairs (untimed instrumented traversal).
@@ -51,5 +63,9 @@ struct Engine {
 };
 
 Engine *make_engine(int use_double);
+
+// Device memory one graph of n nodes and `incidences` incidence entries
+// takes in a batch (an estimate from the engine's own allocations).
+size_t engine_footprint(int use_double, long long n, long long incidences);
 
 } // namespace fa2

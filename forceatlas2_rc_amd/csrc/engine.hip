@@ -59,8 +59,8 @@ struct GraphState {
     double speed, eff, tsw, ttr;
     int done;   // iterations applied
     int status; // 0, or 2 when the totals were not numbers
-    int active; // 1 when this iteration moves the graph
-    int pad;
+    int active;  // 1 when this iteration moves the graph
+    int retired; // 1 once the host took the graph out: no forces computed
 };
 
 struct Tile {
@@ -789,15 +789,16 @@ __device__ inline void lane_pair_adjust(typename Vec<R>::v4 me, typename Vec<R>:
 template <typename R, bool ANY_ADJUST, bool COUNT = false>
 __global__ void __launch_bounds__(256)
 k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
-          const typename Vec<R>::v4 *__restrict__ psort, const TNode<R> *__restrict__ tn,
-          const int *__restrict__ escape, const uint32_t *__restrict__ order,
-          double2 *__restrict__ F, unsigned long long *cnt = nullptr)
+          const GraphState *__restrict__ gs, const typename Vec<R>::v4 *__restrict__ psort,
+          const TNode<R> *__restrict__ tn, const int *__restrict__ escape,
+          const uint32_t *__restrict__ order, double2 *__restrict__ F,
+          unsigned long long *cnt = nullptr)
 {
     using V4 = typename Vec<R>::v4;
     unsigned long long nA = 0, nO = 0, nP = 0;
     const int k0 = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = k0 < N;
-    const int k = valid ? k0 : N - 1;
+    const int k = k0 < N ? k0 : N - 1;
+    const bool valid = k0 < N && !gs[gid[k]].retired;
     const int my_g = valid ? gid[k] : -1;
     const V4 me = psort[k];
     ForceSum<R> sum;
@@ -987,15 +988,16 @@ __device__ inline void attract_terms(int e0, int e1, int step, bool linlog, bool
 template <typename R, int GL>
 __global__ void __launch_bounds__(256)
 k_attract(int count, const int *__restrict__ nodes, const int *__restrict__ gid,
-          const GraphDesc *__restrict__ gd, const typename Vec<R>::v2 *__restrict__ pos,
+          const GraphDesc *__restrict__ gd, const GraphState *__restrict__ gs,
+          const typename Vec<R>::v2 *__restrict__ pos,
           const R *__restrict__ size, const int *__restrict__ inc_off,
           const int *__restrict__ inc_node, const R *__restrict__ inc_coef,
           double2 *__restrict__ F)
 {
     const int slot = blockIdx.x * (256 / GL) + (int) threadIdx.x / GL;
     const int sub = (int) threadIdx.x % GL;
-    const bool live = slot < count;
-    const int i = live ? nodes[slot] : nodes[0];
+    const int i = slot < count ? nodes[slot] : nodes[0];
+    const bool live = slot < count && !gs[gid[i]].retired;
     const int flags = gd[gid[i]].flags;
     KahanSum2<R> sum;
     if (live)
@@ -1020,7 +1022,8 @@ k_attract(int count, const int *__restrict__ nodes, const int *__restrict__ gid,
 template <typename R>
 __global__ void __launch_bounds__(256)
 k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict__ gid,
-                const GraphDesc *__restrict__ gd, const typename Vec<R>::v2 *__restrict__ pos,
+                const GraphDesc *__restrict__ gd, const GraphState *__restrict__ gs,
+                const typename Vec<R>::v2 *__restrict__ pos,
                 const R *__restrict__ size, const int *__restrict__ inc_node,
                 const R *__restrict__ inc_coef, double2 *__restrict__ F,
                 typename Vec<R>::v2 *__restrict__ part)
@@ -1030,6 +1033,7 @@ k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict
     const int lane = lane_id();
     const int4 it = items[w];
     const int i = it.x;
+    if (gs[gid[i]].retired) return;
     const int flags = gd[gid[i]].flags;
     KahanSum2<R> sum;
     attract_terms<R, 1>(it.y + lane, it.z, 64, flags & GF_LINLOG, flags & GF_ADJUST, pos[i], size[i],
@@ -1054,13 +1058,15 @@ k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict
 // One wave per node cut into several items: (node, first slot, slots).
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_attract_fix(int count, const int4 *__restrict__ hubs, const typename Vec<R>::v2 *__restrict__ part,
+k_attract_fix(int count, const int4 *__restrict__ hubs, const int *__restrict__ gid,
+              const GraphState *__restrict__ gs, const typename Vec<R>::v2 *__restrict__ part,
               double2 *__restrict__ F)
 {
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= count) return;
     const int lane = lane_id();
     const int4 h = hubs[w];
+    if (gs[gid[h.x]].retired) return;
     KahanSum2<R> sum;
     for (int q = lane; q < h.z; q += 64) {
         const typename Vec<R>::v2 v = part[h.y + q];
@@ -1124,6 +1130,8 @@ k_swing(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
         part[blockIdx.x] = make_double2(((red[0][0] + red[0][1]) + red[0][2]) + red[0][3],
                                         ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3]);
 }
+
+__global__ void k_retire(GraphState *gs, int g) { gs[g].retired = 1; }
 
 // Device restatement of fa2_update_speed (src/forceatlas2.c:445-485).
 __device__ inline void update_speed(int n, double sw, double tr, double jt_user, double &speed,
@@ -1295,6 +1303,7 @@ template <typename R> struct EngineImpl final : Engine {
     DevBuf<Super> d_supers;
     std::vector<Super> h_supers;
     int NS = 0;
+    bool multi_block = false; // a graph spans several blocks of k_aggregate_tile
     DevBuf<double2> d_part;
 
     // profiling: events between the phases of each iteration
@@ -1302,9 +1311,15 @@ template <typename R> struct EngineImpl final : Engine {
     std::vector<std::vector<hipEvent_t>> ev_iters;
     double stage_s[FA2_FULL_NTIMES] = {0};
 
+    // two iterations (one period of the force buffers) captured once
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
     ~EngineImpl() override
     {
         if (stream) (void) hipStreamSynchronize(stream);
+        if (graph_exec) (void) hipGraphExecDestroy(graph_exec);
+        if (graph) (void) hipGraphDestroy(graph);
         for (hipEvent_t e : ev_pool) (void) hipEventDestroy(e);
     }
 
@@ -1318,7 +1333,10 @@ template <typename R> struct EngineImpl final : Engine {
              int *parent) override;
     int regions(int graph, double *rec, int *skip) override;
     int stage_times(double *t) override;
+    int retire(int graph) override;
+    int progress(int *done, int *status) override;
     int graph_status(int graph) override;
+    bool capture_pair();
     int count_work(unsigned long long out[3]) override;
     int set_state(int graph, const double *pos, const double *fold, double speed,
                   double eff) override;
@@ -1392,7 +1410,11 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     }
     T = toff;
     NS = (int) h_supers.size();
-    for (const GraphDesc &D : h_gd) max_tiles = std::max(max_tiles, D.ntiles);
+    multi_block = false;
+    for (const GraphDesc &D : h_gd) {
+        max_tiles = std::max(max_tiles, D.ntiles);
+        multi_block = multi_block || D.n > agg_block<R>();
+    }
 
     // packed host arrays
     const size_t nn = (size_t) N;
@@ -1451,49 +1473,50 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     }
 
     HipStatus hs;
+    const hipStream_t ast = async_alloc ? stream : nullptr;
     auto up = [&](void *dst, const void *src, size_t bytes) {
         if (hs.ok() && bytes) hs.check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream), "upload");
     };
-    hs.check(d_gd.alloc(G), "alloc");
-    hs.check(d_gs.alloc(G), "alloc");
-    hs.check(d_tiles.alloc(T), "alloc");
-    hs.check(d_gid.alloc(nn), "alloc");
-    hs.check(d_gi.alloc(NI), "alloc");
-    hs.check(d_pos.alloc(nn), "alloc");
-    hs.check(d_node4.alloc(nn), "alloc");
-    hs.check(d_F[0].alloc(nn), "alloc");
-    hs.check(d_F[1].alloc(nn), "alloc");
-    hs.check(d_acc.alloc(Acc<R>::separate ? nn : 1), "alloc");
-    hs.check(d_mass.alloc(nn), "alloc");
-    hs.check(d_size.alloc(nn), "alloc");
-    hs.check(d_fixed.alloc(nn), "alloc");
-    hs.check(d_inc_off.alloc(nn + 1), "alloc");
-    hs.check(d_inc_node.alloc((size_t) I), "alloc");
-    hs.check(d_coef.alloc((size_t) I), "alloc");
-    for (int c = 0; c < 2; c++) hs.check(d_class[c].alloc(h_class[c].size()), "alloc");
-    hs.check(d_items.alloc(h_items.size()), "alloc");
-    hs.check(d_hubs.alloc(h_hubs.size()), "alloc");
-    hs.check(d_hubpart.alloc(nslots), "alloc");
-    hs.check(d_tbox.alloc(T), "alloc");
-    hs.check(d_gbox.alloc(G), "alloc");
-    hs.check(d_psort.alloc(nn + kLeafSize), "alloc");
-    hs.check(d_nd.alloc(2 * (size_t) (NI > 0 ? NI : 1)), "alloc");
-    hs.check(d_keyA.alloc(nn), "alloc");
-    hs.check(d_valA.alloc(nn), "alloc");
-    hs.check(d_keyB.alloc(nn), "alloc");
-    hs.check(d_valB.alloc(nn), "alloc");
-    hs.check(d_hist.alloc(256 * (size_t) T), "alloc");
-    hs.check(d_rowtot.alloc(256 * (size_t) G), "alloc");
-    hs.check(d_child.alloc(NI), "alloc");
-    hs.check(d_tn.alloc(NI), "alloc");
-    hs.check(d_parent_leaf.alloc(nn), "alloc");
-    hs.check(d_parent_int.alloc(NI), "alloc");
-    hs.check(d_escape.alloc(NI), "alloc");
-    hs.check(d_visits.alloc(NI), "alloc");
-    hs.check(d_ready.alloc((size_t) N + NI), "alloc");
-    hs.check(d_rcnt.alloc((size_t) NS + 1), "alloc");
-    hs.check(d_supers.alloc(NS), "alloc");
-    hs.check(d_part.alloc(T), "alloc");
+    hs.check(d_gd.alloc(G, ast), "alloc");
+    hs.check(d_gs.alloc(G, ast), "alloc");
+    hs.check(d_tiles.alloc(T, ast), "alloc");
+    hs.check(d_gid.alloc(nn, ast), "alloc");
+    hs.check(d_gi.alloc(NI, ast), "alloc");
+    hs.check(d_pos.alloc(nn, ast), "alloc");
+    hs.check(d_node4.alloc(nn, ast), "alloc");
+    hs.check(d_F[0].alloc(nn, ast), "alloc");
+    hs.check(d_F[1].alloc(nn, ast), "alloc");
+    hs.check(d_acc.alloc(Acc<R>::separate ? nn : 1, ast), "alloc");
+    hs.check(d_mass.alloc(nn, ast), "alloc");
+    hs.check(d_size.alloc(nn, ast), "alloc");
+    hs.check(d_fixed.alloc(nn, ast), "alloc");
+    hs.check(d_inc_off.alloc(nn + 1, ast), "alloc");
+    hs.check(d_inc_node.alloc((size_t) I, ast), "alloc");
+    hs.check(d_coef.alloc((size_t) I, ast), "alloc");
+    for (int c = 0; c < 2; c++) hs.check(d_class[c].alloc(h_class[c].size(), ast), "alloc");
+    hs.check(d_items.alloc(h_items.size(), ast), "alloc");
+    hs.check(d_hubs.alloc(h_hubs.size(), ast), "alloc");
+    hs.check(d_hubpart.alloc(nslots, ast), "alloc");
+    hs.check(d_tbox.alloc(T, ast), "alloc");
+    hs.check(d_gbox.alloc(G, ast), "alloc");
+    hs.check(d_psort.alloc(nn + kLeafSize, ast), "alloc");
+    hs.check(d_nd.alloc(2 * (size_t) (NI > 0 ? NI : 1), ast), "alloc");
+    hs.check(d_keyA.alloc(nn, ast), "alloc");
+    hs.check(d_valA.alloc(nn, ast), "alloc");
+    hs.check(d_keyB.alloc(nn, ast), "alloc");
+    hs.check(d_valB.alloc(nn, ast), "alloc");
+    hs.check(d_hist.alloc(256 * (size_t) T, ast), "alloc");
+    hs.check(d_rowtot.alloc(256 * (size_t) G, ast), "alloc");
+    hs.check(d_child.alloc(NI, ast), "alloc");
+    hs.check(d_tn.alloc(NI, ast), "alloc");
+    hs.check(d_parent_leaf.alloc(nn, ast), "alloc");
+    hs.check(d_parent_int.alloc(NI, ast), "alloc");
+    hs.check(d_escape.alloc(NI, ast), "alloc");
+    hs.check(d_visits.alloc(NI, ast), "alloc");
+    hs.check(d_ready.alloc((size_t) N + NI, ast), "alloc");
+    hs.check(d_rcnt.alloc((size_t) NS + 1, ast), "alloc");
+    hs.check(d_supers.alloc(NS, ast), "alloc");
+    hs.check(d_part.alloc(T, ast), "alloc");
     if (!hs.ok()) {
         set_hip_error("could not allocate memory on the device", hs.err);
         return 1;
@@ -1591,7 +1614,7 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
                                                      d_parent_leaf.p, d_parent_int.p, d_escape.p,
                                                      NS, N, d_visits.p, d_ready.p, d_rcnt.p,
                                                      d_nd.p, d_tn.p);
-        if (T > G) { // some graph spans several tiles
+        if (multi_block) { // some graph spans several blocks of the first level
             k_aggregate_super<R><<<NS, 1024, 0, stream>>>(d_supers.p, d_gd.p, d_psort.p,
                                                           d_child.p, d_parent_int.p, d_escape.p,
                                                           NS, N, d_visits.p, d_ready.p, d_rcnt.p,
@@ -1607,10 +1630,10 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
     // 5: repulsion
     if (any_adjust)
         k_repulse<R, true><<<ceil_div(N, 256), 256, 0, stream>>>(
-            N, d_gid.p, d_gd.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fc);
+            N, d_gid.p, d_gd.p, d_gs.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fc);
     else
         k_repulse<R, false><<<ceil_div(N, 256), 256, 0, stream>>>(
-            N, d_gid.p, d_gd.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fc);
+            N, d_gid.p, d_gd.p, d_gs.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fc);
     mark();
     // 6: attraction (gravity was applied with the repulsion)
     {
@@ -1618,18 +1641,19 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
         const int ci = (int) h_items.size(), ch = (int) h_hubs.size();
         if (c0)
             k_attract<R, 1><<<ceil_div(c0, 256), 256, 0, stream>>>(
-                c0, d_class[0].p, d_gid.p, d_gd.p, d_pos.p, d_size.p, d_inc_off.p, d_inc_node.p,
+                c0, d_class[0].p, d_gid.p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_off.p, d_inc_node.p,
                 d_coef.p, Fc);
         if (c1)
             k_attract<R, 16><<<ceil_div(c1, 16), 256, 0, stream>>>(
-                c1, d_class[1].p, d_gid.p, d_gd.p, d_pos.p, d_size.p, d_inc_off.p, d_inc_node.p,
+                c1, d_class[1].p, d_gid.p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_off.p, d_inc_node.p,
                 d_coef.p, Fc);
         if (ci)
             k_attract_items<R><<<ceil_div(ci, 4), 256, 0, stream>>>(
-                ci, d_items.p, d_gid.p, d_gd.p, d_pos.p, d_size.p, d_inc_node.p, d_coef.p, Fc,
+                ci, d_items.p, d_gid.p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_node.p, d_coef.p, Fc,
                 d_hubpart.p);
         if (ch)
-            k_attract_fix<R><<<ceil_div(ch, 4), 256, 0, stream>>>(ch, d_hubs.p, d_hubpart.p, Fc);
+            k_attract_fix<R><<<ceil_div(ch, 4), 256, 0, stream>>>(ch, d_hubs.p, d_gid.p, d_gs.p,
+                                                                  d_hubpart.p, Fc);
     }
     mark();
     // 7-8: swinging, speed, displacement
@@ -1641,10 +1665,56 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
     parity = 1 - parity;
 }
 
+// Record two iterations into a HIP graph (no memory set or copy is part of
+// an iteration). False when the capture fails: the iterations are then
+// launched one kernel at a time.
+template <typename R> bool EngineImpl<R>::capture_pair()
+{
+    if (graph_exec) return true;
+    if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+    enqueue_iteration(nullptr);
+    enqueue_iteration(nullptr);
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(stream, &g) != hipSuccess || !g) {
+        (void) hipGetLastError();
+        return false;
+    }
+    if (hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void) hipGraphDestroy(g);
+        graph_exec = nullptr;
+        (void) hipGetLastError();
+        return false;
+    }
+    graph = g;
+    return true;
+}
+
 template <typename R> int EngineImpl<R>::run(int iterations)
 {
     (void) hipSetDevice(device);
     double work = 0.0;
+    if (!profile && use_graph && iterations >= 2) {
+        if (parity != 0) { // the captured pair starts from F[0]
+            enqueue_iteration(nullptr);
+            iterations--;
+        }
+        if (!capture_pair()) use_graph = 0;
+        for (; use_graph && iterations >= 2; iterations -= 2) {
+            work += 2.0 * ((double) N + (double) I);
+            if (work >= 1e6) {
+                work = 0.0;
+                if (interrupted && interrupted()) {
+                    set_message("the layout was interrupted");
+                    return 1;
+                }
+            }
+            const hipError_t e = hipGraphLaunch(graph_exec, stream);
+            if (e != hipSuccess) {
+                set_hip_error("the computation on the GPU failed", e);
+                return 1;
+            }
+        }
+    }
     for (int it = 0; it < iterations; it++) {
         work += (double) N + (double) I;
         if (work >= 1e6) {
@@ -1875,6 +1945,40 @@ template <typename R> int EngineImpl<R>::regions(int graph, double *rec, int *sk
     return 0;
 }
 
+template <typename R> int EngineImpl<R>::retire(int graph)
+{
+    if (graph < 0 || graph >= G) {
+        set_message("no graph %d in this batch", graph);
+        return 1;
+    }
+    (void) hipSetDevice(device);
+    k_retire<<<1, 1, 0, stream>>>(d_gs.p, graph);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_hip_error("could not update the batch", e);
+        return 1;
+    }
+    return 0;
+}
+
+template <typename R> int EngineImpl<R>::progress(int *done, int *status)
+{
+    (void) hipSetDevice(device);
+    std::vector<GraphState> gs(G);
+    hipError_t e = hipMemcpyAsync(gs.data(), d_gs.p, G * sizeof(GraphState), hipMemcpyDeviceToHost,
+                                  stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        set_hip_error("could not read the progress of the batch", e);
+        return 1;
+    }
+    for (int g = 0; g < G; g++) {
+        done[g] = gs[g].done;
+        status[g] = gs[g].status;
+    }
+    return 0;
+}
+
 template <typename R> int EngineImpl<R>::stage_times(double *t)
 {
     for (int k = 0; k < FA2_FULL_NTIMES; k++) t[k] = stage_s[k];
@@ -1893,10 +1997,10 @@ template <typename R> int EngineImpl<R>::count_work(unsigned long long out[3])
     if (e == hipSuccess) {
         if (any_adjust)
             k_repulse<R, true, true><<<ceil_div(N, 256), 256, 0, stream>>>(
-                N, d_gid.p, d_gd.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fs, c.p);
+                N, d_gid.p, d_gd.p, d_gs.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fs, c.p);
         else
             k_repulse<R, false, true><<<ceil_div(N, 256), 256, 0, stream>>>(
-                N, d_gid.p, d_gd.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fs, c.p);
+                N, d_gid.p, d_gd.p, d_gs.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fs, c.p);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, c.p, 3 * sizeof(unsigned long long),
@@ -1954,6 +2058,36 @@ int EngineImpl<R>::set_state(int graph, const double *pos, const double *fold, d
         return 1;
     }
     return 0;
+}
+
+// Device bytes of one graph of n nodes and `incidences` incidence entries,
+// following the allocations of EngineImpl::open (per node, per region, per
+// incidence, per tile; the attraction work lists at their largest).
+template <typename R> static size_t footprint_of(long long n, long long incidences)
+{
+    using V2 = typename Vec<R>::v2;
+    using V4 = typename Vec<R>::v4;
+    const long long ni = n - 1, tiles = (n + kTile - 1) / kTile;
+    const long long items = incidences / kChunk + n / 256 + 2;
+    size_t b = 0;
+    b += (size_t) n * (sizeof(int) /*gid*/ + sizeof(V2) /*pos*/ + sizeof(V4) /*node4*/ +
+                       2 * sizeof(double2) /*F*/ + (Acc<R>::separate ? sizeof(double2) : 0) +
+                       3 * sizeof(R) /*mass, size*/ + 1 /*fixed*/ + sizeof(int) /*inc_off*/ +
+                       sizeof(int) /*class lists*/ + sizeof(V4) /*psort*/ +
+                       4 * sizeof(uint32_t) /*keys, values*/ + sizeof(int) /*parent_leaf*/ +
+                       sizeof(int) /*ready*/);
+    b += (size_t) ni * (sizeof(int) /*gi*/ + 2 * sizeof(V4) /*nd*/ + sizeof(int2) /*child*/ +
+                        sizeof(TNode<R>) + 4 * sizeof(int) /*parent, escape, visits, ready*/);
+    b += (size_t) incidences * (sizeof(int) + sizeof(R));
+    b += (size_t) items * (2 * sizeof(int4) + sizeof(V2));
+    b += (size_t) tiles * (sizeof(Tile) + 2 * sizeof(V4) + sizeof(double2) + 256 * sizeof(uint32_t));
+    b += sizeof(GraphDesc) + sizeof(GraphState) + 256 * sizeof(uint32_t) + 2 * sizeof(Super) + 4096;
+    return b;
+}
+
+size_t engine_footprint(int use_double, long long n, long long incidences)
+{
+    return use_double ? footprint_of<double>(n, incidences) : footprint_of<float>(n, incidences);
 }
 
 Engine *make_engine(int use_double)

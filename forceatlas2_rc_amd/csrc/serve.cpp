@@ -1,0 +1,430 @@
+// fa2_serve_*: a layout server over the batch engine. No counterpart in the
+// reference, whose forceatlas2_large() lays out one graph per call
+// (src/gpu_full.c:164-532) and keeps nothing on the device between calls.
+//
+// Requests are copied in by fa2_serve_submit and wait in arrival order. A
+// worker thread admits them by device-memory footprint: when a stream is
+// free, the request at the head of the queue and every later one of the
+// same precision that fits in what the running batches leave of the budget
+// join one new batch (engine.hip) on that stream.
+// Batches on different streams run at the same time, so requests that
+// arrive while others are on the GPU start as soon as a stream and memory
+// are free, and their work overlaps on the device. Each batch advances in
+// chunks of iterations; after each chunk the worker reads how far every
+// graph has come, fetches the positions of the graphs that reached their
+// iteration count, and retires them (their repulsion and attraction are
+// skipped from then on). A batch's memory, allocated in its stream's order,
+// is released when its last request is done.
+//
+// A request's result does not depend on the other requests: every kernel of
+// the engine computes a graph from that graph's data alone, in a fixed
+// order, so a layout is bit-identical alone and inside any batch.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "common.h"
+#include "engine.h"
+#include "../../include/fa2.h"
+#include "../../include/fa2_hip.h"
+
+using namespace fa2;
+
+namespace {
+
+enum { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3 };
+
+struct Job {
+    int id = 0;
+    fa2_full_params p{};
+    std::vector<int> inc_off, inc_node, fixed;
+    std::vector<double> inc_coef, mass, size, pos;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    size_t bytes = 0;
+    int state = kQueued;
+    std::string error;
+};
+
+struct Running {
+    Engine *e = nullptr;
+    std::vector<Job *> jobs;
+    std::vector<char> finished;
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    size_t bytes = 0;
+    int target = 0; // iterations enqueued so far
+    int longest = 0;
+};
+
+double now_s() { return fa2_now(); }
+
+} // namespace
+
+struct fa2_server {
+    int device = 0, chunk = 25;
+    size_t budget = 0, used = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Job *> queue;
+    std::vector<std::unique_ptr<Job>> jobs;
+    std::vector<Running *> running;
+    std::vector<hipStream_t> streams, free_streams;
+    std::thread worker;
+    bool stop = false;
+    // statistics
+    long long batches = 0, admitted = 0;
+    int peak_running = 0;
+
+    void loop();
+    bool admit(std::unique_lock<std::mutex> &lk);
+    bool advance(Running *r);
+    void finish_batch(Running *r);
+};
+
+// One new batch from the requests at the head of the queue that fit.
+bool fa2_server::admit(std::unique_lock<std::mutex> &lk)
+{
+    if (queue.empty() || free_streams.empty()) return false;
+    // a request that alone exceeds the budget can never run
+    while (!queue.empty() && queue.front()->bytes > budget) {
+        Job *j = queue.front();
+        queue.pop_front();
+        j->state = kFailed;
+        j->error = "the layout needs more device memory than the server's budget";
+        j->t_done = now_s();
+        cv_done.notify_all();
+    }
+    if (queue.empty() || used + queue.front()->bytes > budget) return false;
+    const int use_double = queue.front()->p.use_double;
+    std::vector<Job *> take;
+    size_t bytes = 0;
+    for (auto it = queue.begin(); it != queue.end();) {
+        Job *j = *it;
+        if (j->p.use_double != use_double || used + bytes + j->bytes > budget) {
+            ++it;
+            continue;
+        }
+        bytes += j->bytes;
+        take.push_back(j);
+        it = queue.erase(it);
+    }
+    hipStream_t st = free_streams.back();
+    free_streams.pop_back();
+    used += bytes;
+    lk.unlock();
+
+    // build the batch without holding the lock: the upload takes a while
+    Running *r = new Running();
+    r->jobs = take;
+    r->finished.assign(take.size(), 0);
+    r->st = st;
+    r->bytes = bytes;
+    std::vector<fa2_full_params> ps(take.size());
+    std::vector<const double *> pos(take.size());
+    const double t = now_s();
+    for (size_t k = 0; k < take.size(); k++) {
+        ps[k] = take[k]->p;
+        pos[k] = take[k]->pos.data();
+        take[k]->t_admit = t;
+        r->longest = std::max(r->longest, take[k]->p.iterations);
+    }
+    std::string err;
+    r->e = make_engine(use_double);
+    if (!r->e) {
+        err = "out of memory";
+    } else {
+        r->e->device = device;
+        r->e->stream = st;
+        r->e->async_alloc = 1;
+        std::vector<EngineGraph> gr(take.size());
+        for (size_t k = 0; k < take.size(); k++) gr[k] = EngineGraph{&ps[k], pos[k]};
+        if (r->e->open(gr.data(), (int) take.size()) != 0) err = fa2_gpu_error();
+        else if (hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
+            err = "could not create a HIP event";
+    }
+    if (err.empty()) {
+        r->target = std::min(chunk, r->longest);
+        if (r->e->run(r->target) != 0 || hipEventRecord(r->ev, st) != hipSuccess)
+            err = fa2_gpu_error();
+    }
+
+    lk.lock();
+    if (!err.empty()) {
+        for (Job *j : take) {
+            j->state = kFailed;
+            j->error = err;
+            j->t_done = now_s();
+        }
+        if (r->ev) (void) hipEventDestroy(r->ev);
+        delete r->e;
+        delete r;
+        used -= bytes;
+        free_streams.push_back(st);
+        cv_done.notify_all();
+        return true;
+    }
+    for (Job *j : take) j->state = kRunning;
+    running.push_back(r);
+    batches++;
+    admitted += (long long) take.size();
+    peak_running = std::max(peak_running, (int) running.size());
+    return true;
+}
+
+// After a completed chunk: hand out the finished requests, enqueue the next
+// chunk. True when the batch is over.
+bool fa2_server::advance(Running *r)
+{
+    const int G = (int) r->jobs.size();
+    std::vector<int> done(G), status(G);
+    std::string err;
+    if (r->e->progress(done.data(), status.data()) != 0) err = fa2_gpu_error();
+    bool all = true;
+    for (int k = 0; k < G; k++) {
+        if (r->finished[k]) continue;
+        Job *j = r->jobs[k];
+        if (!err.empty()) {
+            std::lock_guard<std::mutex> lk(mu);
+            j->state = kFailed;
+            j->error = err;
+            j->t_done = now_s();
+            r->finished[k] = 1;
+            continue;
+        }
+        if (status[k] != 0) { // invalid numbers: the graph stopped
+            (void) r->e->retire(k);
+            std::lock_guard<std::mutex> lk(mu);
+            j->state = kFailed;
+            j->error = "the computation on the GPU produced invalid numbers";
+            j->t_done = now_s();
+            r->finished[k] = 1;
+            continue;
+        }
+        if (done[k] < j->p.iterations) {
+            all = false;
+            continue;
+        }
+        std::vector<double> out(2 * (size_t) j->p.n);
+        const int rc = r->e->positions(k, out.data());
+        if (rc == 0) (void) r->e->retire(k);
+        std::lock_guard<std::mutex> lk(mu);
+        if (rc == 0) {
+            j->pos.swap(out);
+            j->state = kDone;
+        } else {
+            j->state = kFailed;
+            j->error = fa2_gpu_error();
+        }
+        j->t_done = now_s();
+        r->finished[k] = 1;
+    }
+    cv_done.notify_all();
+    if (all || !err.empty()) return true;
+    const int step = std::min(chunk, r->longest - r->target);
+    r->target += step;
+    if (r->e->run(step) != 0 || hipEventRecord(r->ev, r->st) != hipSuccess) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (int k = 0; k < G; k++)
+            if (!r->finished[k]) {
+                r->jobs[k]->state = kFailed;
+                r->jobs[k]->error = fa2_gpu_error();
+                r->jobs[k]->t_done = now_s();
+                r->finished[k] = 1;
+            }
+        cv_done.notify_all();
+        return true;
+    }
+    return false;
+}
+
+void fa2_server::finish_batch(Running *r)
+{
+    delete r->e; // waits for its stream, frees in the stream's order
+    (void) hipEventDestroy(r->ev);
+    std::lock_guard<std::mutex> lk(mu);
+    used -= r->bytes;
+    free_streams.push_back(r->st);
+    running.erase(std::find(running.begin(), running.end(), r));
+    delete r;
+}
+
+void fa2_server::loop()
+{
+    (void) hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        if (stop && queue.empty() && running.empty()) break;
+        bool busy = false;
+        while (admit(lk)) busy = true;
+        std::vector<Running *> ready;
+        for (Running *r : running)
+            if (hipEventQuery(r->ev) != hipErrorNotReady) ready.push_back(r);
+        lk.unlock();
+        for (Running *r : ready) {
+            busy = true;
+            if (advance(r)) finish_batch(r);
+        }
+        lk.lock();
+        if (!busy) {
+            if (running.empty())
+                cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+            else // a chunk takes at least tens of microseconds
+                cv_work.wait_for(lk, std::chrono::microseconds(50));
+        }
+    }
+}
+
+extern "C" fa2_server *fa2_serve_open(int device, double budget_bytes, int max_batches,
+                                      int chunk)
+{
+    if (device < 0 || device >= device_count()) {
+        set_message("there is no GPU device %d", device + 1);
+        return nullptr;
+    }
+    if (max_batches < 1 || max_batches > 16 || chunk < 1) {
+        set_message("max_batches must be in 1..16 and chunk at least 1");
+        return nullptr;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        set_message("could not select the GPU device");
+        return nullptr;
+    }
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        set_message("could not read the device memory size");
+        return nullptr;
+    }
+    fa2_server *s = new (std::nothrow) fa2_server();
+    if (!s) {
+        set_message("out of memory");
+        return nullptr;
+    }
+    s->device = device;
+    s->chunk = chunk;
+    // by default 80% of the memory free now
+    s->budget = budget_bytes > 0 ? (size_t) budget_bytes : (size_t) (0.8 * (double) free_b);
+    for (int k = 0; k < max_batches; k++) {
+        hipStream_t st = nullptr;
+        // default priority on purpose: streams of different priorities are
+        // never mixed in this library
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            for (hipStream_t q : s->streams) (void) hipStreamDestroy(q);
+            delete s;
+            set_message("could not create a HIP stream");
+            return nullptr;
+        }
+        s->streams.push_back(st);
+    }
+    s->free_streams = s->streams;
+    s->worker = std::thread([s] { s->loop(); });
+    return s;
+}
+
+extern "C" int fa2_serve_submit(fa2_server *s, const fa2_full_params *p, const double *pos)
+{
+    if (!s || !p || !pos) {
+        set_message("no request");
+        return -1;
+    }
+    if (p->n < 2 || p->n > (1 << 26) || !p->inc_off || !p->mass || p->iterations < 1) {
+        set_message("a request needs 2..2^26 nodes, incidence lists, masses and iterations");
+        return -1;
+    }
+    auto j = std::make_unique<Job>();
+    const int n = p->n, I = p->inc_off[n];
+    j->inc_off.assign(p->inc_off, p->inc_off + n + 1);
+    j->inc_node.assign(p->inc_node, p->inc_node + I);
+    j->inc_coef.assign(p->inc_coef, p->inc_coef + I);
+    j->mass.assign(p->mass, p->mass + n);
+    if (p->adjust && p->size) j->size.assign(p->size, p->size + n);
+    if (p->any_fixed && p->fixed) j->fixed.assign(p->fixed, p->fixed + n);
+    j->pos.assign(pos, pos + 2 * (size_t) n);
+    j->p = *p;
+    j->p.inc_off = j->inc_off.data();
+    j->p.inc_node = j->inc_node.data();
+    j->p.inc_coef = j->inc_coef.data();
+    j->p.mass = j->mass.data();
+    j->p.size = j->size.empty() ? nullptr : j->size.data();
+    j->p.fixed = j->fixed.empty() ? nullptr : j->fixed.data();
+    j->p.device = s->device;
+    j->p.use_double = p->use_double ? 1 : 0;
+    j->p.profile = 0;
+    j->p.interrupted = nullptr;
+    j->bytes = engine_footprint(j->p.use_double, n, I);
+    std::lock_guard<std::mutex> lk(s->mu);
+    j->id = (int) s->jobs.size();
+    j->t_submit = now_s();
+    s->queue.push_back(j.get());
+    s->jobs.push_back(std::move(j));
+    s->cv_work.notify_all();
+    return s->jobs.back()->id;
+}
+
+extern "C" int fa2_serve_status(fa2_server *s, int job)
+{
+    if (!s) return -1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (job < 0 || job >= (int) s->jobs.size()) return -1;
+    return s->jobs[job]->state;
+}
+
+extern "C" int fa2_serve_wait(fa2_server *s, int job, double *pos, double times[3])
+{
+    if (!s) {
+        set_message("no server");
+        return 1;
+    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (job < 0 || job >= (int) s->jobs.size()) {
+        set_message("no request %d on this server", job);
+        return 1;
+    }
+    Job *j = s->jobs[job].get();
+    s->cv_done.wait(lk, [&] { return j->state == kDone || j->state == kFailed; });
+    if (times) {
+        times[0] = j->t_admit > 0 ? j->t_admit - j->t_submit : 0.0;
+        times[1] = j->t_admit > 0 ? j->t_done - j->t_admit : 0.0;
+        times[2] = j->t_done - j->t_submit;
+    }
+    if (j->state == kFailed) {
+        set_message("%s", j->error.c_str());
+        return 1;
+    }
+    if (pos) std::copy(j->pos.begin(), j->pos.end(), pos);
+    return 0;
+}
+
+extern "C" int fa2_serve_stats(fa2_server *s, double out[4])
+{
+    if (!s) return 1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    out[0] = (double) s->batches;
+    out[1] = (double) s->admitted;
+    out[2] = (double) s->peak_running;
+    out[3] = (double) s->budget;
+    return 0;
+}
+
+extern "C" void fa2_serve_close(fa2_server *s)
+{
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_work.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    for (hipStream_t st : s->streams) (void) hipStreamDestroy(st);
+    delete s;
+}
+
+extern "C" double fa2_serve_footprint(int use_double, int n, long long incidences)
+{
+    return (double) engine_footprint(use_double, n, incidences);
+}

@@ -70,6 +70,32 @@ def barabasi_albert(n, m=3, seed=0):
     return n, edges[:ne].astype(np.int32)
 
 
+def preferential_attachment(n, m=3, seed=0):
+    """Preferential attachment for large n, vectorised by generations: the
+    nodes of a generation (a quarter of the nodes so far) each draw m
+    earlier nodes with probability proportional to degree + 1 as it was
+    when the generation began; repeated draws of one node collapse into one
+    edge. Starts from a clique of m + 1 nodes."""
+    rng = np.random.default_rng(seed)
+    k0 = min(n, m + 1)
+    a, b = np.tril_indices(k0, -1)
+    edges = [np.stack([a, b], 1)]
+    urn = np.concatenate([a, b, np.arange(k0)])
+    t = k0
+    while t < n:
+        c = min(n - t, max(64, t // 4))
+        new = np.arange(t, t + c)
+        picks = urn[rng.integers(0, urn.size, size=(c, m))]
+        e = np.stack([np.repeat(new, m), picks.ravel()], 1)
+        key = e[:, 0].astype(np.int64) * n + e[:, 1]
+        _, idx = np.unique(key, return_index=True)
+        e = e[np.sort(idx)]
+        edges.append(e)
+        urn = np.concatenate([urn, e[:, 0], e[:, 1], new])
+        t += c
+    return n, np.concatenate(edges).astype(np.int32)
+
+
 def rmat(scale=21, n_edges=16_000_000, a=0.57, b=0.19, c=0.19, seed=0,
          chunk=1 << 22):
     """R-MAT graph on 2^scale nodes: edges drawn quadrant by quadrant with

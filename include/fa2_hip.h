@@ -115,6 +115,43 @@ void *fa2_batch_stream(fa2_batch *b);
 
 void fa2_batch_close(fa2_batch *b);
 
+/* ----- server ----------------------------------------------------------- */
+
+typedef struct fa2_server fa2_server;
+
+/* A layout server on one device: requests are admitted by device-memory
+ * footprint (budget_bytes, or 80% of the free memory when <= 0) into
+ * batches of one precision each, up to max_batches (1..16) of them running
+ * at once on their own streams, each advanced `chunk` iterations at a
+ * time. NULL on error. */
+fa2_server *fa2_serve_open(int device, double budget_bytes, int max_batches, int chunk);
+
+/* Queue one layout (arguments as for fa2_full_run, use_double choosing its
+ * precision; the device and profile fields are the server's) from the
+ * initial positions pos (x then y). The inputs are copied. Returns the
+ * request id, or -1. */
+int fa2_serve_submit(fa2_server *s, const fa2_full_params *p, const double *pos);
+
+/* 0 queued, 1 running, 2 done, 3 failed; -1 for an unknown request. */
+int fa2_serve_status(fa2_server *s, int job);
+
+/* Wait for a request: its final positions (x then y, may be NULL) and
+ * times (may be NULL) = (seconds queued, seconds from admission to the
+ * result, seconds from submission to the result). 0 on success, 1 when the
+ * request failed (message in fa2_gpu_error()). */
+int fa2_serve_wait(fa2_server *s, int job, double *pos, double times[3]);
+
+/* out = (batches formed, requests admitted, most batches running at once,
+ * memory budget in bytes). */
+int fa2_serve_stats(fa2_server *s, double out[4]);
+
+/* Finish the queued and running requests, then release the server. */
+void fa2_serve_close(fa2_server *s);
+
+/* Device bytes a request of n nodes and `incidences` incidence entries
+ * takes in a batch: the footprint used for admission. */
+double fa2_serve_footprint(int use_double, int n, long long incidences);
+
 #ifdef __cplusplus
 }
 #endif
