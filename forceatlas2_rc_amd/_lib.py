@@ -48,6 +48,13 @@ class DevInfo(Structure):
                 ("memory", c_double)]
 
 
+class Edges(Structure):
+    """fa2_edges (include/fa2_hip.h): an edge list for the preprocessing on
+    the device."""
+    _fields_ = [("m", c_int), ("frm", POINTER(c_int)), ("to", POINTER(c_int)),
+                ("w", POINTER(c_double)), ("outbound", c_int)]
+
+
 class FullParams(Structure):
     """fa2_full_params (src/fa2.h:97-115)."""
     _fields_ = [("n", c_int), ("inc_off", POINTER(c_int)),
@@ -92,6 +99,9 @@ PROTOTYPES = {
                               c_void_p, c_void_p, c_int]),
     "fa2_batch_open": (c_void_p, [c_int, POINTER(FullParams),
                                   POINTER(c_void_p)]),
+    "fa2_batch_open_edges": (c_void_p, [c_int, POINTER(FullParams), POINTER(Edges),
+                                        POINTER(c_void_p)]),
+    "fa2_batch_csr": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fa2_batch_run": (c_int, [c_void_p, c_int]),
     "fa2_batch_sync": (c_int, [c_void_p]),
     "fa2_batch_positions": (c_int, [c_void_p, c_int, c_void_p]),
@@ -108,6 +118,8 @@ PROTOTYPES = {
     "fa2_batch_close": (None, [c_void_p]),
     "fa2_serve_open": (c_void_p, [c_int, c_double, c_int, c_int]),
     "fa2_serve_submit": (c_int, [c_void_p, POINTER(FullParams), c_void_p]),
+    "fa2_serve_submit_edges": (c_int, [c_void_p, POINTER(FullParams), POINTER(Edges),
+                                       c_void_p]),
     "fa2_serve_status": (c_int, [c_void_p, c_int]),
     "fa2_serve_wait": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "fa2_serve_stats": (c_int, [c_void_p, c_void_p]),

@@ -17,7 +17,7 @@ from ctypes import POINTER, byref, c_double, c_int, c_uint32, c_void_p
 
 import numpy as np
 
-from ._lib import (FA2_FULL_NTIMES, DevInfo, FullParams, GPUError,
+from ._lib import (FA2_FULL_NTIMES, DevInfo, Edges, FullParams, GPUError,
                    INTERRUPT_FN, last_error, lib)
 
 __all__ = ["gpu_devices", "layout_forceatlas2_gpu", "layout_forceatlas2_large",
@@ -284,23 +284,34 @@ def prepare_large(n, frm, to, w, outbound):
     return mass, inc_off, inc_node, inc_coef
 
 
-def _fill_params(p, gr, device, precision, profile):
+def _fill_params(p, gr, device, precision, profile, edges=None):
     """Fill the fa2_full_params p from the graph dict gr (see Batch); returns
     the arrays p points into (to be kept alive) and the positions (x then
-    y)."""
+    y). With an Edges structure `edges`, the edge list goes there for the
+    preprocessing on the device, and p's mass and incidence lists stay
+    empty."""
     n = int(gr["n"])
     if "frm" in gr:
-        frm = np.asarray(gr["frm"], np.int32)
-        to = np.asarray(gr["to"], np.int32)
+        frm = np.ascontiguousarray(gr["frm"], np.int32)
+        to = np.ascontiguousarray(gr["to"], np.int32)
     else:
         frm, to = _edges(n, gr.get("edges"))
-    w = np.ones(frm.size) if gr.get("weights") is None \
-        else np.asarray(gr["weights"], np.float64)
+        frm = np.ascontiguousarray(frm, np.int32)
+        to = np.ascontiguousarray(to, np.int32)
+    w = None if gr.get("weights") is None else np.ascontiguousarray(gr["weights"], np.float64)
     outbound = bool(gr.get("dissuade_hubs", False))
-    if "inc_off" in gr:
+    if edges is not None:
+        edges.m = int(frm.size)
+        edges.frm = frm.ctypes.data_as(POINTER(c_int))
+        edges.to = to.ctypes.data_as(POINTER(c_int))
+        edges.w = w.ctypes.data_as(POINTER(c_double)) if w is not None else None
+        edges.outbound = int(outbound)
+        mass = off = node = coef = None
+    elif "inc_off" in gr:
         mass, off, node, coef = (gr["mass"], gr["inc_off"], gr["inc_node"], gr["inc_coef"])
     else:
-        mass, off, node, coef = prepare_large(n, frm, to, w, outbound)
+        mass, off, node, coef = prepare_large(n, frm, to, np.ones(frm.size) if w is None else w,
+                                              outbound)
     adjust = bool(gr.get("adjust", False))
     size = np.ascontiguousarray(
         np.broadcast_to(np.asarray(gr.get("size", 1.0), np.float64), (n,)))
@@ -309,10 +320,11 @@ def _fill_params(p, gr, device, precision, profile):
     init = np.asarray(gr["init"], np.float64)
     xy = np.ascontiguousarray(init.T).reshape(-1).copy()
     p.n = n
-    p.inc_off = off.ctypes.data_as(POINTER(c_int))
-    p.inc_node = node.ctypes.data_as(POINTER(c_int))
-    p.inc_coef = coef.ctypes.data_as(POINTER(c_double))
-    p.mass = mass.ctypes.data_as(POINTER(c_double))
+    if off is not None:
+        p.inc_off = off.ctypes.data_as(POINTER(c_int))
+        p.inc_node = node.ctypes.data_as(POINTER(c_int))
+        p.inc_coef = coef.ctypes.data_as(POINTER(c_double))
+        p.mass = mass.ctypes.data_as(POINTER(c_double))
     p.size = size.ctypes.data_as(POINTER(c_double))
     p.fixed = fixed.ctypes.data_as(POINTER(c_int))
     p.any_fixed = int(fixed.any())
@@ -329,7 +341,7 @@ def _fill_params(p, gr, device, precision, profile):
     p.profile = int(bool(profile))
     p.source = None
     p.interrupted = INTERRUPT_FN(0)
-    return [mass, off, node, coef, size, fixed, xy], xy
+    return [mass, off, node, coef, size, fixed, xy, frm, to, w], xy
 
 
 class Batch:
@@ -339,21 +351,31 @@ class Batch:
     weights (already transformed, default 1), init ((n, 2)), iterations, and
     the settings scaling, gravity, theta, jitter_tolerance, strong, linlog,
     dissuade_hubs, adjust, size, fixed. The device (0-based), the precision
-    and ``profile`` are common to the batch.
+    and ``profile`` are common to the batch. ``ingest`` chooses where the
+    mass and incidence lists are built: "host" (numpy, or the prepared
+    mass/inc_off/inc_node/inc_coef of the dicts) or "device"
+    (fa2_batch_open_edges).
     """
 
-    def __init__(self, graphs, device=0, precision="single", profile=False):
+    def __init__(self, graphs, device=0, precision="single", profile=False, ingest="host"):
+        if ingest not in ("host", "device"):
+            raise ValueError("ingest must be 'host' or 'device'")
         self._keep = []
         G = len(graphs)
         params = (FullParams * G)()
+        edges = (Edges * G)() if ingest == "device" else None
         pos_ptrs = (c_void_p * G)()
         self.n = []
         for g, gr in enumerate(graphs):
-            keep, xy = _fill_params(params[g], gr, device, precision, profile)
+            keep, xy = _fill_params(params[g], gr, device, precision, profile,
+                                    edges[g] if edges is not None else None)
             pos_ptrs[g] = xy.ctypes.data
             self._keep += keep
             self.n.append(int(gr["n"]))
-        self.h = lib.fa2_batch_open(G, params, pos_ptrs)
+        if edges is not None:
+            self.h = lib.fa2_batch_open_edges(G, params, edges, pos_ptrs)
+        else:
+            self.h = lib.fa2_batch_open(G, params, pos_ptrs)
         self._keep = None
         if not self.h:
             raise GPUError(last_error())
@@ -400,6 +422,21 @@ class Batch:
         return dict(keys=ks, order=order, child=child, range=rng_,
                     parent=parent)
 
+    def csr(self, g=0):
+        """The mass and incidence lists of graph g as the device holds them:
+        mass (n), inc_off (n + 1), inc_node, inc_coef (in the batch's
+        precision, as float64)."""
+        n = self.n[g]
+        off = np.empty(n + 1, np.int32)
+        if lib.fa2_batch_csr(self.h, g, None, _ptr(off), None, None) != 0:
+            raise GPUError(last_error())
+        mass = np.empty(n)
+        node = np.empty(int(off[-1]), np.int32)
+        coef = np.empty(int(off[-1]))
+        if lib.fa2_batch_csr(self.h, g, _ptr(mass), _ptr(off), _ptr(node), _ptr(coef)) != 0:
+            raise GPUError(last_error())
+        return mass, off, node, coef
+
     def regions(self, g=0):
         """Traversal records of graph g's regions in the reference numbering:
         an (n-1, 4) array of (centre x, centre y, mass, size^2) and the skip
@@ -429,9 +466,9 @@ class Batch:
             raise GPUError(last_error())
 
     def count_work(self):
-        """(accepted regions, opened regions, node pairs) of the repulsion on
-        the last tree."""
-        out = np.zeros(3, np.uint64)
+        """(accepted regions, opened regions, node pairs, wave steps) of the
+        repulsion on the last tree."""
+        out = np.zeros(4, np.uint64)
         if lib.fa2_batch_count_work(self.h, _ptr(out)) != 0:
             raise GPUError(last_error())
         return tuple(int(v) for v in out)
@@ -476,12 +513,21 @@ class Server:
         self.device = int(device)
         self.n = {}
 
-    def submit(self, graph, precision="single"):
+    def submit(self, graph, precision="single", ingest="host"):
+        """Queue one layout; ``ingest`` as for Batch ("device": the edge
+        list is sent and prepared on the device)."""
         if precision not in ("single", "double"):
             raise ValueError("precision must be 'single' or 'double'")
+        if ingest not in ("host", "device"):
+            raise ValueError("ingest must be 'host' or 'device'")
         p = FullParams()
-        keep, xy = _fill_params(p, graph, self.device, precision, False)
-        job = lib.fa2_serve_submit(self.h, ctypes.byref(p), xy.ctypes.data)
+        e = Edges() if ingest == "device" else None
+        keep, xy = _fill_params(p, graph, self.device, precision, False, e)
+        if e is not None:
+            job = lib.fa2_serve_submit_edges(self.h, ctypes.byref(p), ctypes.byref(e),
+                                             xy.ctypes.data)
+        else:
+            job = lib.fa2_serve_submit(self.h, ctypes.byref(p), xy.ctypes.data)
         del keep
         if job < 0:
             raise GPUError(last_error())

@@ -16,8 +16,34 @@ struct fa2_batch {
     ~fa2_batch() { delete e; }
 };
 
+static fa2_batch *open_batch(int ngraphs, const fa2_full_params *params,
+                             const fa2_edges *edges, const double *const *pos);
+
 extern "C" fa2_batch *fa2_batch_open(int ngraphs, const fa2_full_params *params,
                                      const double *const *pos)
+{
+    return open_batch(ngraphs, params, nullptr, pos);
+}
+
+extern "C" fa2_batch *fa2_batch_open_edges(int ngraphs, const fa2_full_params *params,
+                                           const fa2_edges *edges, const double *const *pos)
+{
+    if (!edges) {
+        set_message("no edge lists");
+        return nullptr;
+    }
+    return open_batch(ngraphs, params, edges, pos);
+}
+
+extern "C" int fa2_batch_csr(fa2_batch *b, int g, double *mass, int *inc_off, int *inc_node,
+                             double *inc_coef)
+{
+    if (!b) return 1;
+    return b->e->csr(g, mass, inc_off, inc_node, inc_coef);
+}
+
+static fa2_batch *open_batch(int ngraphs, const fa2_full_params *params,
+                             const fa2_edges *edges, const double *const *pos)
 {
     if (ngraphs < 1 || !params || !pos) {
         set_message("no graph to lay out");
@@ -45,6 +71,7 @@ extern "C" fa2_batch *fa2_batch_open(int ngraphs, const fa2_full_params *params,
     for (int g = 0; g < ngraphs; g++) {
         gr[g].p = &params[g];
         gr[g].pos = pos[g];
+        gr[g].edges = edges ? &edges[g] : nullptr;
     }
     if (b->e->open(gr.data(), ngraphs)) {
         delete b;
@@ -102,7 +129,7 @@ extern "C" int fa2_batch_stage_times(fa2_batch *b, double *times)
     return b->e->stage_times(times);
 }
 
-extern "C" int fa2_batch_count_work(fa2_batch *b, unsigned long long out[3])
+extern "C" int fa2_batch_count_work(fa2_batch *b, unsigned long long out[4])
 {
     if (!b) return 1;
     return b->e->count_work(out);

@@ -7,6 +7,7 @@
 #include <cstdint>
 
 #include "../../include/fa2.h"
+#include "../../include/fa2_hip.h"
 
 namespace fa2 {
 
@@ -15,6 +16,9 @@ namespace fa2 {
 struct EngineGraph {
     const fa2_full_params *p;
     const double *pos;
+    // the edge list, when the mass and incidence lists are to be built on
+    // the device (p's mass and inc_* are then not read); else null
+    const fa2_edges *edges = nullptr;
 };
 
 struct Engine {
@@ -44,6 +48,9 @@ struct Engine {
     // end). Either pointer may be NULL.
     virtual int regions(int graph, double *rec, int *skip) = 0;
     virtual int stage_times(double *t) = 0;
+    // The incidence lists of one graph as held on the device, in the graph's
+    // numbering (see fa2_batch_csr).
+    virtual int csr(int graph, double *mass, int *inc_off, int *inc_node, double *inc_coef) = 0;
     // Stop computing graph `graph` (its positions and forces stay as they
     // are): the batch keeps its slot, but its repulsion and attraction are
     // skipped from the next iteration on.
@@ -54,8 +61,8 @@ struct Engine {
     virtual int progress(int *done, int *status) = 0;
     virtual int graph_status(int graph) = 0;
     // Work of the repulsion on the current tree: regions accepted, regions
-    // opened, node pairs (untimed instrumented traversal).
-    virtual int count_work(unsigned long long out[3]) = 0;
+    // opened, node pairs, wave steps (untimed instrumented traversal).
+    virtual int count_work(unsigned long long out[4]) = 0;
     // Replace the state of one graph: positions (x then y), previous force
     // (x then y, NULL for zero), speed and speed efficiency.
     virtual int set_state(int graph, const double *pos, const double *fold, double speed,

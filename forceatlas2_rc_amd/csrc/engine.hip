@@ -19,7 +19,7 @@
 //               [noff_g - g, noff_g - g + n_g - 1)
 //   CSR    inc_off (N+1), inc_node (I), inc_coef (I, R)
 // Node references inside a tree: r >= 0 a region, r < 0 the leaf at sorted
-// position ~r, kEnd the end of a traversal.
+// position ~r. The traversal's own links are record indices (record_of).
 //
 // Every reduction is made in a fixed order: results are reproducible run
 // to run, and a graph gives the same result alone and in a batch as long
@@ -36,11 +36,16 @@
 #include "common.h"
 #include "kernels_common.h"
 #include "engine.h"
+#include "ingest.h"
 #include "../../include/fa2.h"
 
 namespace fa2 {
 
-constexpr int kEnd = INT_MIN;
+// The traversal reads one record per region and per leaf: region r at r,
+// the leaf at sorted position p at NI + p. Its links (left child, skip) are
+// record indices, and kNoRecord ends a walk.
+constexpr int kNoRecord = -1;
+__device__ inline int record_of(int r, int NI) { return r >= 0 ? r : NI + ~r; }
 constexpr int kLeafSize = 8; // regions of at most 8 nodes are visited node by node
 
 // ---------------------------------------------------------------------------
@@ -450,7 +455,7 @@ __global__ void k_karras(int NI, const int *__restrict__ gi, const GraphDesc *__
     else parent_leaf[~right] = t;
     if (i == 0) parent_int[t] = -1;
     escape[S - g] = right;
-    tn[t].l = make_int4(left, 0, base + first, base + last);
+    tn[t].l = make_int4(record_of(left, NI), 0, base + first, base + last);
     visits[t] = 0;
     for (int q = t; q <= NS; q += NI) rcnt[q] = 0;
 }
@@ -495,7 +500,7 @@ template <typename R> __device__ inline typename Vec<R>::v4 region_record(const 
 }
 
 template <typename R>
-__device__ inline void merge_region(int t, int g, int gend,
+__device__ inline void merge_region(int t, int g, int gend, int NI,
                                     const typename Vec<R>::v4 *__restrict__ psort,
                                     const int2 *__restrict__ child,
                                     const int *__restrict__ escape,
@@ -520,7 +525,7 @@ __device__ inline void merge_region(int t, int g, int gend,
     nd[2 * t + 1] = mk4<R>(r.m, r.mx, r.my, (R) 0);
     tn[t].c = region_record<R>(r);
     const int last = tn[t].l.w;
-    tn[t].l.y = last == gend - 1 ? kEnd : escape[last - g];
+    tn[t].l.y = last == gend - 1 ? kNoRecord : record_of(escape[last - g], NI);
 }
 
 // Leaf to root in three levels, the second child to arrive at a region
@@ -573,8 +578,8 @@ __global__ void __launch_bounds__(1024)
 k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
                  const typename Vec<R>::v4 *__restrict__ psort, const int2 *__restrict__ child,
                  const int *__restrict__ parent_leaf, const int *__restrict__ parent_int,
-                 const int *__restrict__ escape, int NS, int N, int *gvis, int *ready, int *rcnt,
-                 typename Vec<R>::v4 *nd, TNode<R> *tn)
+                 const int *__restrict__ escape, int NS, int N, int NI, int *gvis, int *ready,
+                 int *rcnt, typename Vec<R>::v4 *nd, TNode<R> *tn)
 {
     using V4 = typename Vec<R>::v4;
     constexpr int B = agg_block<R>();
@@ -595,8 +600,20 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     // internal ids stop at noff + n - 1 - g - 1: the last position of a
     // graph has no region of its own index
     const int nslots = (hi == gend) ? hi - lo - 1 : hi - lo;
+    const bool adjust = D.flags & GF_ADJUST;
     for (int q = threadIdx.x; q < hi - lo; q += blockDim.x) {
-        s_leaf[q] = psort[lo + q];
+        const V4 leaf = psort[lo + q];
+        s_leaf[q] = leaf;
+        // the leaf's own traversal record (NI + sorted position): a region of
+        // one node and size 0. Accepted as a monopole it gives exactly the
+        // pair term without sizes; with sizes it is never accepted (size^2
+        // infinite) and is visited as a pair.
+        const int pos = lo + q;
+        TNode<R> rec;
+        rec.c = mk4<R>(leaf.x, leaf.y, leaf.z, adjust ? (R) INFINITY : (R) 0);
+        rec.l = make_int4(0, pos == gend - 1 ? kNoRecord : record_of(escape[pos - g], NI), pos,
+                          pos);
+        tn[(size_t) NI + pos] = rec;
         int vis = kOutside;
         if (q < nslots) {
             const int t = tbase + q;
@@ -668,7 +685,7 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
         if (s_vis[q] >= kOutside) continue;
         const int t = tbase + q;
         const int last = tn[t].l.w;
-        tn[t].l.y = last == gend - 1 ? kEnd : escape[last - g];
+        tn[t].l.y = last == gend - 1 ? kNoRecord : record_of(escape[last - g], NI);
     }
 }
 
@@ -677,7 +694,8 @@ __global__ void __launch_bounds__(1024)
 k_aggregate_super(const Super *__restrict__ supers, const GraphDesc *__restrict__ gd,
                   const typename Vec<R>::v4 *__restrict__ psort, const int2 *__restrict__ child,
                   const int *__restrict__ parent_int, const int *__restrict__ escape, int NS,
-                  int N, int *gvis, int *ready, int *rcnt, typename Vec<R>::v4 *nd, TNode<R> *tn)
+                  int N, int NI, int *gvis, int *ready, int *rcnt, typename Vec<R>::v4 *nd,
+                  TNode<R> *tn)
 {
     const Super S = supers[blockIdx.x];
     const int g = S.g;
@@ -687,7 +705,7 @@ k_aggregate_super(const Super *__restrict__ supers, const GraphDesc *__restrict_
     for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
         int t = ready[S.lo + q];
         for (int depth = 0; depth < 128; depth++) {
-            merge_region<R>(t, g, gend, psort, child, escape, nd, tn);
+            merge_region<R>(t, g, gend, NI, psort, child, escape, nd, tn);
             const int p = parent_int[t];
             if (p < 0) break;
             const int4 l = tn[p].l;
@@ -708,7 +726,7 @@ __global__ void __launch_bounds__(1024)
 k_aggregate_top(const GraphDesc *__restrict__ gd, const typename Vec<R>::v4 *__restrict__ psort,
                 const int2 *__restrict__ child, const int *__restrict__ parent_int,
                 const int *__restrict__ escape, const int *__restrict__ gi, int NS, int N,
-                const int *__restrict__ ready, const int *__restrict__ rcnt, int *gvis,
+                int NI, const int *__restrict__ ready, const int *__restrict__ rcnt, int *gvis,
                 typename Vec<R>::v4 *nd, TNode<R> *tn)
 {
     const int cnt = rcnt[NS];
@@ -717,7 +735,7 @@ k_aggregate_top(const GraphDesc *__restrict__ gd, const typename Vec<R>::v4 *__r
         const int g = gi[t];
         const int gend = gd[g].noff + gd[g].n;
         for (int depth = 0; depth < 128; depth++) {
-            merge_region<R>(t, g, gend, psort, child, escape, nd, tn);
+            merge_region<R>(t, g, gend, NI, psort, child, escape, nd, tn);
             const int p = parent_int[t];
             if (p < 0) break;
             const int old = __hip_atomic_fetch_add(&gvis[p], 1, __ATOMIC_ACQ_REL,
@@ -786,16 +804,15 @@ __device__ inline void lane_pair_adjust(typename Vec<R>::v4 me, typename Vec<R>:
 // accepted (A) and opened (O), and node pairs (P), into cnt[0..2].
 // psort holds kLeafSize records of padding after the last node, so that the
 // nodes of a small region are loaded at once.
-template <typename R, bool ANY_ADJUST, bool COUNT = false>
+template <typename R, bool ANY_ADJUST, bool COUNT = false, bool WIDE = false>
 __global__ void __launch_bounds__(256)
-k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
+k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
           const GraphState *__restrict__ gs, const typename Vec<R>::v4 *__restrict__ psort,
-          const TNode<R> *__restrict__ tn, const int *__restrict__ escape,
-          const uint32_t *__restrict__ order, double2 *__restrict__ F,
-          unsigned long long *cnt = nullptr)
+          const TNode<R> *__restrict__ tn, const uint32_t *__restrict__ order,
+          double2 *__restrict__ F, unsigned long long *cnt = nullptr)
 {
     using V4 = typename Vec<R>::v4;
-    unsigned long long nA = 0, nO = 0, nP = 0;
+    unsigned long long nA = 0, nO = 0, nP = 0, nS = 0;
     const int k0 = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = k0 < N ? k0 : N - 1;
     const bool valid = k0 < N && !gs[gid[k]].retired;
@@ -818,51 +835,54 @@ k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
         int c = G.ioff;                          // the root (every graph has n >= 2)
         // every node of the tree is visited at most once (2n - 1 of them); the
         // bound only guards against a corrupt tree
-        for (int steps = 2 * G.n; c != kEnd && steps > 0; steps--) {
+        // One exit, at the loop's head: a walk ends at kNoRecord. It always
+        // ends: a descent goes to a left child, whose range lies strictly
+        // inside its parent's, and a skip link to the first node after the
+        // current range, so the walk is a depth-first pass that meets every
+        // node of the tree at most once. (k_karras builds a valid binary
+        // radix tree from any array of keys, whatever the positions.)
+        while (c >= 0) {
             c = __builtin_amdgcn_readfirstlane(c);
-            // lane masks of the step, kept in scalar registers: the
-            // comparisons write them directly and the lanes are selected
-            // from them without per-lane selects
-            int j0, j1, nxt;
-            uint64_t tm; // lanes that visit the step's nodes one by one
-            if (c >= 0) {
-                const TNode<R> node = tn[c]; // one scalar load of the whole record
-                const V4 rc = node.c;
-                const int4 rl = node.l;
-                if (sizeof(R) == 4) // keep the eight dwords together in SGPRs
-                    asm volatile("" ::"s"(rc.x), "s"(rc.y), "s"(rc.z), "s"(rc.w), "s"(rl.x),
-                                 "s"(rl.y), "s"(rl.z), "s"(rl.w));
-                const R xd = me.x - rc.x, yd = me.y - rc.y;
-                const R d2 = xd * xd + yd * yd;
-                const uint64_t am = __builtin_amdgcn_ballot_w64(rl.z > done);
-                const uint64_t fm = __builtin_amdgcn_ballot_w64(d2 * theta2 > rc.w);
-                if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {
-                    // the monopole, for the lanes that accept: a branch on
-                    // the lane mask (the lanes' sums are updated in place)
-                    asm volatile("");
-                    sum.add(xd, yd, rc.z * frcp_r(d2));
-                    done = rl.w;
-                    if (COUNT) nA++;
-                }
-                tm = am & ~fm;
-                if (tm == 0) { // every lane is done with this region
-                    c = rl.y;
-                    continue;
-                }
-                if (COUNT) nO += __builtin_amdgcn_inverse_ballot_w64(tm);
-                if (rl.w - rl.z >= kLeafSize) { // descend
-                    c = rl.x;
-                    continue;
-                }
-                j0 = rl.z; // a small region: its nodes one by one
-                j1 = rl.w;
-                nxt = rl.y;
-            } else {
-                j0 = j1 = ~c;
-                tm = __builtin_amdgcn_ballot_w64(j0 > done);
-                nxt = j0 == gend - 1 ? kEnd : escape[j0 - g];
+            if (COUNT) nS++;
+            // c is the record of a region or a leaf; lane masks of the step
+            // are kept in scalar registers: the comparisons write them
+            // directly and the lanes are selected from them without per-lane
+            // selects. The record is one scalar load, at a 32-bit byte offset
+            // while the records take less than 4 GiB.
+            const TNode<R> node =
+                WIDE ? tn[c]
+                     : *(const TNode<R> *) ((const char *) tn +
+                                            (uint32_t) c * (uint32_t) sizeof(TNode<R>));
+            const V4 rc = node.c;
+            const int4 rl = node.l;
+            if (sizeof(R) == 4) // keep the eight dwords together in SGPRs
+                asm volatile("" ::"s"(rc.x), "s"(rc.y), "s"(rc.z), "s"(rc.w), "s"(rl.x), "s"(rl.y),
+                             "s"(rl.z), "s"(rl.w));
+            const R xd = me.x - rc.x, yd = me.y - rc.y;
+            const R d2 = xd * xd + yd * yd;
+            const uint64_t am = __builtin_amdgcn_ballot_w64(rl.z > done);
+            const uint64_t fm = __builtin_amdgcn_ballot_w64(d2 * theta2 > rc.w);
+            if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {
+                // the monopole, for the lanes that accept: a branch on the
+                // lane mask (the lanes' sums are updated in place)
+                asm volatile("");
+                sum.add(xd, yd, rc.z * frcp_r(d2));
+                done = rl.w;
+                if (COUNT) nA++;
             }
-            // up to kLeafSize nodes, loaded together (psort is padded)
+            const uint64_t tm = am & ~fm; // lanes that open the region
+            if (tm == 0) { // every lane is done with this region
+                c = rl.y;
+                continue;
+            }
+            if (COUNT) nO += __builtin_amdgcn_inverse_ballot_w64(tm) && rl.z != rl.w;
+            if (rl.w - rl.z >= kLeafSize) { // descend
+                c = rl.x;
+                continue;
+            }
+            // a small region or a leaf: its nodes one by one, loaded together
+            // (psort is padded)
+            const int j0 = rl.z, j1 = rl.w;
             V4 o[kLeafSize];
 #pragma unroll
             for (int u = 0; u < kLeafSize; u++) o[u] = psort[j0 + u];
@@ -883,15 +903,15 @@ k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
                     if (adjust) {
                         lane_pair_adjust<R>(me, o[u], sum, true);
                     } else {
-                        const R xd = me.x - o[u].x, yd = me.y - o[u].y;
-                        const R d2 = xd * xd + yd * yd;
-                        const R w = d2 > (R) 0 ? o[u].z * frcp_r(d2) : (R) 0;
-                        sum.add(xd, yd, w);
+                        const R pxd = me.x - o[u].x, pyd = me.y - o[u].y;
+                        const R pd2 = pxd * pxd + pyd * pyd;
+                        const R w = pd2 > (R) 0 ? o[u].z * frcp_r(pd2) : (R) 0;
+                        sum.add(pxd, pyd, w);
                     }
                 }
                 done = j1;
             }
-            c = nxt;
+            c = rl.y;
         }
     }
     if (valid) {
@@ -915,6 +935,7 @@ k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
         atomicAdd(&cnt[0], nA);
         atomicAdd(&cnt[1], nO);
         atomicAdd(&cnt[2], nP);
+        if (lane_id() == 0) atomicAdd(&cnt[3], nS);
     }
 }
 
@@ -930,28 +951,41 @@ k_repulse(int N, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
 
 constexpr int kChunk = 1024;
 
+// log1p(d) / d of the linlog attraction. In single precision it is
+// computed as log(u) / (u - 1) with u = 1 + d: for u != 1 this equals
+// log1p(d) / d up to the roundings of log(u), of u - 1 and of the
+// reciprocal (Kahan's log1p identity), with the hardware's v_log_f32 (log2)
+// and v_rcp_f32; for u = 1 the ratio is 1. In double precision it is the
+// library's log1p and an IEEE division.
+__device__ inline float linlog_factor(float d)
+{
+    const float u = 1.0f + d;
+    if (u == 1.0f) return 1.0f;
+    return (__builtin_amdgcn_logf(u) * 0.693147180559945309f) * __builtin_amdgcn_rcpf(u - 1.0f);
+}
+__device__ inline double linlog_factor(double d) { return log1p(d) / d; }
+
+// The attraction of one incidence (pipeline.cl:412-427): weight
+// w = -c, times log1p(d) / d with linlog; with sizes the distance is the gap
+// between the discs, computed in FP64 as for the repulsion, and the edge only
+// pulls while it is positive. The term w * (xd, yd) goes into the FP64 sum.
 template <typename R>
 __device__ inline void attract_term(typename Vec<R>::v2 pi, typename Vec<R>::v2 pj, R c, R si,
-                                    R sj, bool linlog, bool adjust, KahanSum2<R> &sum)
+                                    R sj, bool linlog, bool adjust, ForceSum<R> &sum)
 {
     const R xd = pi.x - pj.x, yd = pi.y - pj.y;
-    R f = -c;
+    R w = -c;
     if (adjust) {
-        // the gap between the discs, in FP64 as for the repulsion: the edge
-        // only pulls while it is positive
         const double dx = (double) xd, dy = (double) yd;
         const double gap = __builtin_sqrt(dx * dx + dy * dy) - ((double) si + (double) sj);
         if (!(gap > 0.0)) return;
-        if (linlog) {
-            const R dist = (R) gap;
-            f *= flog1p_r(dist) / dist;
-        }
+        if (linlog) w *= linlog_factor((R) gap);
     } else if (linlog) {
         const R dist = fsqrt_r(xd * xd + yd * yd);
         if (!(dist > (R) 0)) return;
-        f *= flog1p_r(dist) / dist;
+        w *= linlog_factor(dist);
     }
-    sum.add(xd * f, yd * f);
+    sum.add(xd, yd, w);
 }
 
 // The incidences e0, e0 + step, ... < e1 of one node, in that order; the
@@ -961,7 +995,7 @@ __device__ inline void attract_terms(int e0, int e1, int step, bool linlog, bool
                                      typename Vec<R>::v2 pi, R si,
                                      const typename Vec<R>::v2 *__restrict__ pos,
                                      const R *__restrict__ size, const int *__restrict__ inc_node,
-                                     const R *__restrict__ inc_coef, KahanSum2<R> &sum)
+                                     const R *__restrict__ inc_coef, ForceSum<R> &sum)
 {
     for (int e = e0; e < e1; e += U * step) {
         int j[U];
@@ -985,6 +1019,15 @@ __device__ inline void attract_terms(int e0, int e1, int step, bool linlog, bool
     }
 }
 
+template <int GL> __device__ inline void group_sum(double &x, double &y)
+{
+#pragma unroll
+    for (int o = GL / 2; o >= 1; o >>= 1) {
+        x += __shfl_xor(x, o, 64);
+        y += __shfl_xor(y, o, 64);
+    }
+}
+
 template <typename R, int GL>
 __global__ void __launch_bounds__(256)
 k_attract(int count, const int *__restrict__ nodes, const int *__restrict__ gid,
@@ -999,22 +1042,18 @@ k_attract(int count, const int *__restrict__ nodes, const int *__restrict__ gid,
     const int i = slot < count ? nodes[slot] : nodes[0];
     const bool live = slot < count && !gs[gid[i]].retired;
     const int flags = gd[gid[i]].flags;
-    KahanSum2<R> sum;
+    ForceSum<R> sum;
     if (live)
         // a lane per node: four incidences in flight per lane; 16 lanes per
         // node already keep enough loads in flight
         attract_terms<R, GL == 1 ? 4 : 1>(inc_off[i] + sub, inc_off[i + 1], GL, flags & GF_LINLOG,
-                         flags & GF_ADJUST, pos[i], size[i], pos, size, inc_node, inc_coef, sum);
-    R ax = sum.sum_x(), ay = sum.sum_y();
-#pragma unroll
-    for (int o = GL / 2; o >= 1; o >>= 1) {
-        ax += __shfl_xor(ax, o, 64);
-        ay += __shfl_xor(ay, o, 64);
-    }
+                                          flags & GF_ADJUST, pos[i], size[i], pos, size, inc_node,
+                                          inc_coef, sum);
+    group_sum<GL>(sum.x, sum.y);
     if (!live || sub != 0) return;
     double2 f = F[i];
-    f.x += (double) ax;
-    f.y += (double) ay;
+    f.x += sum.x;
+    f.y += sum.y;
     F[i] = f;
 }
 
@@ -1026,7 +1065,7 @@ k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict
                 const typename Vec<R>::v2 *__restrict__ pos,
                 const R *__restrict__ size, const int *__restrict__ inc_node,
                 const R *__restrict__ inc_coef, double2 *__restrict__ F,
-                typename Vec<R>::v2 *__restrict__ part)
+                double2 *__restrict__ part)
 {
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= count) return;
@@ -1035,23 +1074,20 @@ k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict
     const int i = it.x;
     if (gs[gid[i]].retired) return;
     const int flags = gd[gid[i]].flags;
-    KahanSum2<R> sum;
-    attract_terms<R, 1>(it.y + lane, it.z, 64, flags & GF_LINLOG, flags & GF_ADJUST, pos[i], size[i],
-                     pos, size, inc_node, inc_coef, sum);
-    R ax = sum.sum_x(), ay = sum.sum_y();
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        ax += __shfl_xor(ax, o, 64);
-        ay += __shfl_xor(ay, o, 64);
-    }
+    ForceSum<R> sum;
+    // one incidence in flight per lane: unrolled, the wave items measured
+    // slower (more registers, fewer waves resident)
+    attract_terms<R, 1>(it.y + lane, it.z, 64, flags & GF_LINLOG, flags & GF_ADJUST, pos[i],
+                        size[i], pos, size, inc_node, inc_coef, sum);
+    group_sum<64>(sum.x, sum.y);
     if (lane != 0) return;
     if (it.w < 0) {
         double2 f = F[i];
-        f.x += (double) ax;
-        f.y += (double) ay;
+        f.x += sum.x;
+        f.y += sum.y;
         F[i] = f;
     } else {
-        part[it.w] = mk2<R>(ax, ay);
+        part[it.w] = make_double2(sum.x, sum.y);
     }
 }
 
@@ -1059,7 +1095,7 @@ k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict
 template <typename R>
 __global__ void __launch_bounds__(256)
 k_attract_fix(int count, const int4 *__restrict__ hubs, const int *__restrict__ gid,
-              const GraphState *__restrict__ gs, const typename Vec<R>::v2 *__restrict__ part,
+              const GraphState *__restrict__ gs, const double2 *__restrict__ part,
               double2 *__restrict__ F)
 {
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1067,21 +1103,17 @@ k_attract_fix(int count, const int4 *__restrict__ hubs, const int *__restrict__ 
     const int lane = lane_id();
     const int4 h = hubs[w];
     if (gs[gid[h.x]].retired) return;
-    KahanSum2<R> sum;
+    double ax = 0.0, ay = 0.0;
     for (int q = lane; q < h.z; q += 64) {
-        const typename Vec<R>::v2 v = part[h.y + q];
-        sum.add(v.x, v.y);
+        const double2 v = part[h.y + q];
+        ax += v.x;
+        ay += v.y;
     }
-    R ax = sum.sum_x(), ay = sum.sum_y();
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        ax += __shfl_xor(ax, o, 64);
-        ay += __shfl_xor(ay, o, 64);
-    }
+    group_sum<64>(ax, ay);
     if (lane != 0) return;
     double2 f = F[h.x];
-    f.x += (double) ax;
-    f.y += (double) ay;
+    f.x += ax;
+    f.y += ay;
     F[h.x] = f;
 }
 
@@ -1292,7 +1324,7 @@ template <typename R> struct EngineImpl final : Engine {
     DevBuf<int> d_inc_off, d_inc_node;
     DevBuf<int> d_class[2];
     DevBuf<int4> d_items, d_hubs;
-    DevBuf<V2> d_hubpart;
+    DevBuf<double2> d_hubpart;
     DevBuf<V4> d_tbox, d_gbox, d_psort, d_nd;
     DevBuf<uint32_t> d_keyA, d_valA, d_keyB, d_valB, d_hist, d_rowtot;
     DevBuf<int2> d_child;
@@ -1333,15 +1365,36 @@ template <typename R> struct EngineImpl final : Engine {
              int *parent) override;
     int regions(int graph, double *rec, int *skip) override;
     int stage_times(double *t) override;
+    int csr(int graph, double *mass, int *inc_off, int *inc_node, double *inc_coef) override;
     int retire(int graph) override;
     int progress(int *done, int *status) override;
     int graph_status(int graph) override;
     bool capture_pair();
-    int count_work(unsigned long long out[3]) override;
+    int count_work(unsigned long long out[4]) override;
     int set_state(int graph, const double *pos, const double *fold, double speed,
                   double eff) override;
 
     void enqueue_iteration(std::vector<hipEvent_t> *ev);
+
+    // the repulsion kernel for this batch: with sizes or not, counting or
+    // not, record offsets in 32 or 64 bits
+    template <bool COUNT> void launch_repulse(double2 *Fout, unsigned long long *cnt)
+    {
+        const bool wide = ((size_t) NI + (size_t) N) * sizeof(TNode<R>) >= ((size_t) 1 << 32);
+        const unsigned blocks = ceil_div(N, 256);
+#define FA2_REPULSE(ADJ, WIDE)                                                                  \
+    k_repulse<R, ADJ, COUNT, WIDE><<<blocks, 256, 0, stream>>>(N, NI, d_gid.p, d_gd.p, d_gs.p,   \
+                                                               d_psort.p, d_tn.p, d_valA.p, Fout, \
+                                                               cnt)
+        if (any_adjust) {
+            if (wide) FA2_REPULSE(true, true);
+            else FA2_REPULSE(true, false);
+        } else {
+            if (wide) FA2_REPULSE(false, true);
+            else FA2_REPULSE(false, false);
+        }
+#undef FA2_REPULSE
+    }
 };
 
 template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs)
@@ -1352,6 +1405,7 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         return 1;
     }
     // graph descriptors, tiles
+    const bool from_edges = gr[0].edges != nullptr;
     long long nsum = 0, isum = 0;
     for (int g = 0; g < G; g++) {
         const fa2_full_params *p = gr[g].p;
@@ -1359,8 +1413,14 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
             set_message("the all-GPU layout handles between 2 and 67 million nodes");
             return 1;
         }
+        if ((gr[g].edges != nullptr) != from_edges ||
+            (from_edges && (gr[g].edges->m < 0 || (gr[g].edges->m > 0 &&
+                                                   (!gr[g].edges->from || !gr[g].edges->to))))) {
+            set_message("every graph of a batch needs its edge list, or none does");
+            return 1;
+        }
         nsum += p->n;
-        isum += p->inc_off[p->n];
+        isum += from_edges ? 2LL * gr[g].edges->m : p->inc_off[p->n];
     }
     if (nsum > INT_MAX / 2 || isum > INT_MAX - 1) {
         set_message("too many nodes or edges for one batch");
@@ -1419,8 +1479,8 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     // packed host arrays
     const size_t nn = (size_t) N;
     std::vector<int> gid(nn), gi(NI > 0 ? NI : 1), inc_off(nn + 1);
-    std::vector<int> inc_node(I > 0 ? I : 1);
-    std::vector<R> coef(I > 0 ? I : 1), mass(nn), size(nn);
+    std::vector<int> inc_node(from_edges ? 1 : (I > 0 ? I : 1));
+    std::vector<R> coef(from_edges ? 1 : (I > 0 ? I : 1)), mass(from_edges ? 1 : nn), size(nn);
     std::vector<V2> pos(nn);
     std::vector<double2> acc(Acc<R>::separate ? nn : 1);
     std::vector<uint8_t> fixed(nn);
@@ -1432,14 +1492,17 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         for (int i = 0; i < p->n; i++) {
             const size_t q = (size_t) D.noff + i;
             gid[q] = g;
-            mass[q] = (R) p->mass[i];
             size[q] = (p->adjust && p->size) ? (R) p->size[i] : (R) 0;
             fixed[q] = (p->any_fixed && p->fixed) ? (p->fixed[i] != 0) : 0;
             pos[q] = mk2<R>((R) x[i], (R) y[i]);
             if (Acc<R>::separate) acc[q] = make_double2(x[i], y[i]);
-            inc_off[q] = (int) (ioff + p->inc_off[i]);
+            if (!from_edges) {
+                mass[q] = (R) p->mass[i];
+                inc_off[q] = (int) (ioff + p->inc_off[i]);
+            }
         }
         for (int i = 0; i < D.n - 1; i++) gi[(size_t) D.ioff + i] = g;
+        if (from_edges) continue;
         const int ni = p->inc_off[p->n];
         for (int e = 0; e < ni; e++) {
             const int j = p->inc_node[e];
@@ -1453,6 +1516,26 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         ioff += ni;
     }
     inc_off[nn] = (int) ioff;
+    const hipStream_t ast = async_alloc ? stream : nullptr;
+    if (from_edges) {
+        // mass, incidence lists and coefficients on the device; only the
+        // row offsets come back, for the attraction work lists
+        std::vector<IngestGraph> ig(G);
+        std::vector<const fa2_edges *> ed(G);
+        long long eo = 0;
+        for (int g = 0; g < G; g++) {
+            const fa2_edges *E = gr[g].edges;
+            const long long n = h_gd[g].n;
+            // the mean mass: its sum n + 2m is an integer, exact in FP64
+            ig[g] = IngestGraph{eo, h_gd[g].noff, h_gd[g].n, E->outbound ? 1 : 0, 0,
+                                E->outbound ? (double) (n + 2LL * E->m) / (double) n : 1.0};
+            ed[g] = E;
+            eo += E->m;
+        }
+        if (ingest_graphs<R>(stream, ast, N, ig, ed, d_mass, d_inc_off, d_inc_node, d_coef, inc_off))
+            return 1;
+        I = inc_off[nn];
+    }
     // attraction work: one lane per node up to 16 incidences, 16 lanes up
     // to 256, then waves over chunks of kChunk incidences
     for (int i = 0; i < N; i++) {
@@ -1473,7 +1556,6 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     }
 
     HipStatus hs;
-    const hipStream_t ast = async_alloc ? stream : nullptr;
     auto up = [&](void *dst, const void *src, size_t bytes) {
         if (hs.ok() && bytes) hs.check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream), "upload");
     };
@@ -1487,12 +1569,14 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     hs.check(d_F[0].alloc(nn, ast), "alloc");
     hs.check(d_F[1].alloc(nn, ast), "alloc");
     hs.check(d_acc.alloc(Acc<R>::separate ? nn : 1, ast), "alloc");
-    hs.check(d_mass.alloc(nn, ast), "alloc");
+    if (!from_edges) {
+        hs.check(d_mass.alloc(nn, ast), "alloc");
+        hs.check(d_inc_off.alloc(nn + 1, ast), "alloc");
+        hs.check(d_inc_node.alloc((size_t) I, ast), "alloc");
+        hs.check(d_coef.alloc((size_t) I, ast), "alloc");
+    }
     hs.check(d_size.alloc(nn, ast), "alloc");
     hs.check(d_fixed.alloc(nn, ast), "alloc");
-    hs.check(d_inc_off.alloc(nn + 1, ast), "alloc");
-    hs.check(d_inc_node.alloc((size_t) I, ast), "alloc");
-    hs.check(d_coef.alloc((size_t) I, ast), "alloc");
     for (int c = 0; c < 2; c++) hs.check(d_class[c].alloc(h_class[c].size(), ast), "alloc");
     hs.check(d_items.alloc(h_items.size(), ast), "alloc");
     hs.check(d_hubs.alloc(h_hubs.size(), ast), "alloc");
@@ -1508,7 +1592,7 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     hs.check(d_hist.alloc(256 * (size_t) T, ast), "alloc");
     hs.check(d_rowtot.alloc(256 * (size_t) G, ast), "alloc");
     hs.check(d_child.alloc(NI, ast), "alloc");
-    hs.check(d_tn.alloc(NI, ast), "alloc");
+    hs.check(d_tn.alloc((size_t) NI + nn, ast), "alloc"); // regions, then one record per leaf
     hs.check(d_parent_leaf.alloc(nn, ast), "alloc");
     hs.check(d_parent_int.alloc(NI, ast), "alloc");
     hs.check(d_escape.alloc(NI, ast), "alloc");
@@ -1534,16 +1618,24 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     up(d_gid.p, gid.data(), nn * sizeof(int));
     if (NI > 0) up(d_gi.p, gi.data(), (size_t) NI * sizeof(int));
     up(d_pos.p, pos.data(), nn * sizeof(V2));
-    std::vector<V4> node4(nn);
-    for (size_t i = 0; i < nn; i++) node4[i] = mk4<R>(pos[i].x, pos[i].y, mass[i], size[i]);
-    up(d_node4.p, node4.data(), nn * sizeof(V4));
     if (Acc<R>::separate) up(d_acc.p, acc.data(), nn * sizeof(double2));
-    up(d_mass.p, mass.data(), nn * sizeof(R));
     up(d_size.p, size.data(), nn * sizeof(R));
     up(d_fixed.p, fixed.data(), nn);
-    up(d_inc_off.p, inc_off.data(), (nn + 1) * sizeof(int));
-    up(d_inc_node.p, inc_node.data(), (size_t) I * sizeof(int));
-    up(d_coef.p, coef.data(), (size_t) I * sizeof(R));
+    if (from_edges) {
+        if (hs.ok()) {
+            k_pack_node4<R><<<ceil_div(N, 256), 256, 0, stream>>>(N, d_pos.p, d_mass.p, d_size.p,
+                                                                  d_node4.p);
+            hs.check(hipGetLastError(), "launch");
+        }
+    } else {
+        std::vector<V4> node4(nn);
+        for (size_t i = 0; i < nn; i++) node4[i] = mk4<R>(pos[i].x, pos[i].y, mass[i], size[i]);
+        up(d_node4.p, node4.data(), nn * sizeof(V4));
+        up(d_mass.p, mass.data(), nn * sizeof(R));
+        up(d_inc_off.p, inc_off.data(), (nn + 1) * sizeof(int));
+        up(d_inc_node.p, inc_node.data(), (size_t) I * sizeof(int));
+        up(d_coef.p, coef.data(), (size_t) I * sizeof(R));
+    }
     for (int c = 0; c < 2; c++) up(d_class[c].p, h_class[c].data(), h_class[c].size() * sizeof(int));
     up(d_items.p, h_items.data(), h_items.size() * sizeof(int4));
     up(d_hubs.p, h_hubs.data(), h_hubs.size() * sizeof(int4));
@@ -1610,30 +1702,24 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
         k_karras<R><<<ceil_div(NI, 256), 256, 0, stream>>>(
             NI, d_gi.p, d_gd.p, d_keyA.p, d_child.p, d_tn.p, d_parent_leaf.p, d_parent_int.p,
             d_escape.p, d_visits.p, NS, d_rcnt.p);
-        k_aggregate_tile<R><<<T * (kTile / agg_block<R>()), 1024, 0, stream>>>(d_tiles.p, d_gd.p, d_psort.p, d_child.p,
-                                                     d_parent_leaf.p, d_parent_int.p, d_escape.p,
-                                                     NS, N, d_visits.p, d_ready.p, d_rcnt.p,
-                                                     d_nd.p, d_tn.p);
+        k_aggregate_tile<R><<<T * (kTile / agg_block<R>()), 1024, 0, stream>>>(
+            d_tiles.p, d_gd.p, d_psort.p, d_child.p, d_parent_leaf.p, d_parent_int.p, d_escape.p,
+            NS, N, NI, d_visits.p, d_ready.p, d_rcnt.p, d_nd.p, d_tn.p);
         if (multi_block) { // some graph spans several blocks of the first level
             k_aggregate_super<R><<<NS, 1024, 0, stream>>>(d_supers.p, d_gd.p, d_psort.p,
                                                           d_child.p, d_parent_int.p, d_escape.p,
-                                                          NS, N, d_visits.p, d_ready.p, d_rcnt.p,
-                                                          d_nd.p, d_tn.p);
+                                                          NS, N, NI, d_visits.p, d_ready.p,
+                                                          d_rcnt.p, d_nd.p, d_tn.p);
             if (NS > G) // and some graph several super-tiles
                 k_aggregate_top<R><<<1, 1024, 0, stream>>>(d_gd.p, d_psort.p, d_child.p,
                                                             d_parent_int.p, d_escape.p, d_gi.p, NS,
-                                                            N, d_ready.p, d_rcnt.p, d_visits.p,
+                                                            N, NI, d_ready.p, d_rcnt.p, d_visits.p,
                                                             d_nd.p, d_tn.p);
         }
     }
     mark();
     // 5: repulsion
-    if (any_adjust)
-        k_repulse<R, true><<<ceil_div(N, 256), 256, 0, stream>>>(
-            N, d_gid.p, d_gd.p, d_gs.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fc);
-    else
-        k_repulse<R, false><<<ceil_div(N, 256), 256, 0, stream>>>(
-            N, d_gid.p, d_gd.p, d_gs.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fc);
+    launch_repulse<false>(Fc, nullptr);
     mark();
     // 6: attraction (gravity was applied with the repulsion)
     {
@@ -1938,9 +2024,48 @@ template <typename R> int EngineImpl<R>::regions(int graph, double *rec, int *sk
             rec[4 * r + 3] = (double) tn[r].c.w;
         }
         if (skip) {
-            const int s = tn[r].l.y;
-            skip[r] = s == kEnd ? -1 : s >= 0 ? s - D.ioff : n - 1 + (~s - D.noff);
+            const int s = tn[r].l.y; // a record index
+            skip[r] = s < 0 ? -1 : s < NI ? s - D.ioff : n - 1 + (s - NI - D.noff);
         }
+    }
+    return 0;
+}
+
+template <typename R>
+int EngineImpl<R>::csr(int graph, double *mass, int *inc_off, int *inc_node, double *inc_coef)
+{
+    if (graph < 0 || graph >= G) {
+        set_message("no graph %d in this batch", graph);
+        return 1;
+    }
+    (void) hipSetDevice(device);
+    const GraphDesc &D = h_gd[graph];
+    const size_t n = (size_t) D.n;
+    std::vector<int> off(n + 1);
+    std::vector<R> m(n);
+    hipError_t e = hipStreamSynchronize(stream);
+    auto down = [&](void *dst, const void *src, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    };
+    down(off.data(), d_inc_off.p + D.noff, (n + 1) * sizeof(int));
+    down(m.data(), d_mass.p + D.noff, n * sizeof(R));
+    const size_t i0 = (size_t) off[0], ni = (size_t) (off[n] - off[0]);
+    std::vector<int> node(ni);
+    std::vector<R> coef(ni);
+    down(node.data(), d_inc_node.p + i0, ni * sizeof(int));
+    down(coef.data(), d_coef.p + i0, ni * sizeof(R));
+    if (e != hipSuccess) {
+        set_hip_error("could not fetch the incidence lists from the device", e);
+        return 1;
+    }
+    for (size_t i = 0; i < n; i++) {
+        if (mass) mass[i] = (double) m[i];
+        if (inc_off) inc_off[i] = off[i] - (int) i0;
+    }
+    if (inc_off) inc_off[n] = off[n] - (int) i0;
+    for (size_t k = 0; k < ni; k++) {
+        if (inc_node) inc_node[k] = node[k] - D.noff;
+        if (inc_coef) inc_coef[k] = (double) coef[k];
     }
     return 0;
 }
@@ -1985,25 +2110,20 @@ template <typename R> int EngineImpl<R>::stage_times(double *t)
     return 0;
 }
 
-template <typename R> int EngineImpl<R>::count_work(unsigned long long out[3])
+template <typename R> int EngineImpl<R>::count_work(unsigned long long out[4])
 {
     (void) hipSetDevice(device);
     DevBuf<unsigned long long> c;
-    hipError_t e = c.alloc(3);
-    if (e == hipSuccess) e = hipMemsetAsync(c.p, 0, 3 * sizeof(unsigned long long), stream);
+    hipError_t e = c.alloc(4);
+    if (e == hipSuccess) e = hipMemsetAsync(c.p, 0, 4 * sizeof(unsigned long long), stream);
     // the tree of the last iteration; the forces go to the buffer that the
     // next iteration overwrites first
     double2 *Fs = d_F[parity].p;
     if (e == hipSuccess) {
-        if (any_adjust)
-            k_repulse<R, true, true><<<ceil_div(N, 256), 256, 0, stream>>>(
-                N, d_gid.p, d_gd.p, d_gs.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fs, c.p);
-        else
-            k_repulse<R, false, true><<<ceil_div(N, 256), 256, 0, stream>>>(
-                N, d_gid.p, d_gd.p, d_gs.p, d_psort.p, d_tn.p, d_escape.p, d_valA.p, Fs, c.p);
+        launch_repulse<true>(Fs, c.p);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(out, c.p, 3 * sizeof(unsigned long long),
+    if (e == hipSuccess) e = hipMemcpyAsync(out, c.p, 4 * sizeof(unsigned long long),
                                             hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) {
@@ -2078,8 +2198,9 @@ template <typename R> static size_t footprint_of(long long n, long long incidenc
                        sizeof(int) /*ready*/);
     b += (size_t) ni * (sizeof(int) /*gi*/ + 2 * sizeof(V4) /*nd*/ + sizeof(int2) /*child*/ +
                         sizeof(TNode<R>) + 4 * sizeof(int) /*parent, escape, visits, ready*/);
+    b += (size_t) n * sizeof(TNode<R>); // the leaves' traversal records
     b += (size_t) incidences * (sizeof(int) + sizeof(R));
-    b += (size_t) items * (2 * sizeof(int4) + sizeof(V2));
+    b += (size_t) items * (2 * sizeof(int4) + sizeof(double2));
     b += (size_t) tiles * (sizeof(Tile) + 2 * sizeof(V4) + sizeof(double2) + 256 * sizeof(uint32_t));
     b += sizeof(GraphDesc) + sizeof(GraphState) + 256 * sizeof(uint32_t) + 2 * sizeof(Super) + 4096;
     return b;

@@ -2,11 +2,13 @@
 // reference, whose forceatlas2_large() lays out one graph per call
 // (src/gpu_full.c:164-532) and keeps nothing on the device between calls.
 //
-// Requests are copied in by fa2_serve_submit and wait in arrival order. A
-// worker thread admits them by device-memory footprint: when a stream is
-// free, the request at the head of the queue and every later one of the
-// same precision that fits in what the running batches leave of the budget
-// join one new batch (engine.hip) on that stream.
+// Requests are copied in by fa2_serve_submit (prepared incidence lists) or
+// fa2_serve_submit_edges (edge lists, prepared on the device) and wait in
+// arrival order. A worker thread admits them by device-memory footprint:
+// when a stream is free, the request at the head of the queue and every
+// later one of the same precision and kind that fits in what the running
+// batches leave of the budget join one new batch (engine.hip) on that
+// stream.
 // Batches on different streams run at the same time, so requests that
 // arrive while others are on the GPU start as soon as a stream and memory
 // are free, and their work overlaps on the device. Each batch advances in
@@ -45,6 +47,11 @@ struct Job {
     fa2_full_params p{};
     std::vector<int> inc_off, inc_node, fixed;
     std::vector<double> inc_coef, mass, size, pos;
+    // an edge list instead of incidence lists: prepared on the device
+    bool has_edges = false;
+    fa2_edges edges{};
+    std::vector<int> from, to;
+    std::vector<double> w;
     double t_submit = 0, t_admit = 0, t_done = 0;
     size_t bytes = 0;
     int state = kQueued;
@@ -102,11 +109,13 @@ bool fa2_server::admit(std::unique_lock<std::mutex> &lk)
     }
     if (queue.empty() || used + queue.front()->bytes > budget) return false;
     const int use_double = queue.front()->p.use_double;
+    const bool has_edges = queue.front()->has_edges;
     std::vector<Job *> take;
     size_t bytes = 0;
     for (auto it = queue.begin(); it != queue.end();) {
         Job *j = *it;
-        if (j->p.use_double != use_double || used + bytes + j->bytes > budget) {
+        if (j->p.use_double != use_double || j->has_edges != has_edges ||
+            used + bytes + j->bytes > budget) {
             ++it;
             continue;
         }
@@ -143,7 +152,8 @@ bool fa2_server::admit(std::unique_lock<std::mutex> &lk)
         r->e->stream = st;
         r->e->async_alloc = 1;
         std::vector<EngineGraph> gr(take.size());
-        for (size_t k = 0; k < take.size(); k++) gr[k] = EngineGraph{&ps[k], pos[k]};
+        for (size_t k = 0; k < take.size(); k++)
+            gr[k] = EngineGraph{&ps[k], pos[k], has_edges ? &take[k]->edges : nullptr};
         if (r->e->open(gr.data(), (int) take.size()) != 0) err = fa2_gpu_error();
         else if (hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
             err = "could not create a HIP event";
@@ -326,37 +336,54 @@ extern "C" fa2_server *fa2_serve_open(int device, double budget_bytes, int max_b
     return s;
 }
 
-extern "C" int fa2_serve_submit(fa2_server *s, const fa2_full_params *p, const double *pos)
+static int submit(fa2_server *s, const fa2_full_params *p, const fa2_edges *e, const double *pos)
 {
     if (!s || !p || !pos) {
         set_message("no request");
         return -1;
     }
-    if (p->n < 2 || p->n > (1 << 26) || !p->inc_off || !p->mass || p->iterations < 1) {
-        set_message("a request needs 2..2^26 nodes, incidence lists, masses and iterations");
+    if (p->n < 2 || p->n > (1 << 26) || p->iterations < 1 ||
+        (e ? (e->m < 0 || (e->m > 0 && (!e->from || !e->to))) : (!p->inc_off || !p->mass))) {
+        set_message("a request needs 2..2^26 nodes, its edges or incidence lists, and "
+                    "iterations");
         return -1;
     }
     auto j = std::make_unique<Job>();
-    const int n = p->n, I = p->inc_off[n];
-    j->inc_off.assign(p->inc_off, p->inc_off + n + 1);
-    j->inc_node.assign(p->inc_node, p->inc_node + I);
-    j->inc_coef.assign(p->inc_coef, p->inc_coef + I);
-    j->mass.assign(p->mass, p->mass + n);
+    const int n = p->n;
+    long long I = 0;
+    if (e) {
+        j->has_edges = true;
+        j->from.assign(e->from, e->from + e->m);
+        j->to.assign(e->to, e->to + e->m);
+        if (e->w) j->w.assign(e->w, e->w + e->m);
+        j->edges = fa2_edges{e->m, j->from.data(), j->to.data(), e->w ? j->w.data() : nullptr,
+                             e->outbound};
+        I = 2LL * e->m;
+    } else {
+        I = p->inc_off[n];
+        j->inc_off.assign(p->inc_off, p->inc_off + n + 1);
+        j->inc_node.assign(p->inc_node, p->inc_node + I);
+        j->inc_coef.assign(p->inc_coef, p->inc_coef + I);
+        j->mass.assign(p->mass, p->mass + n);
+    }
     if (p->adjust && p->size) j->size.assign(p->size, p->size + n);
     if (p->any_fixed && p->fixed) j->fixed.assign(p->fixed, p->fixed + n);
     j->pos.assign(pos, pos + 2 * (size_t) n);
     j->p = *p;
-    j->p.inc_off = j->inc_off.data();
-    j->p.inc_node = j->inc_node.data();
-    j->p.inc_coef = j->inc_coef.data();
-    j->p.mass = j->mass.data();
+    j->p.inc_off = e ? nullptr : j->inc_off.data();
+    j->p.inc_node = e ? nullptr : j->inc_node.data();
+    j->p.inc_coef = e ? nullptr : j->inc_coef.data();
+    j->p.mass = e ? nullptr : j->mass.data();
     j->p.size = j->size.empty() ? nullptr : j->size.data();
     j->p.fixed = j->fixed.empty() ? nullptr : j->fixed.data();
     j->p.device = s->device;
     j->p.use_double = p->use_double ? 1 : 0;
     j->p.profile = 0;
     j->p.interrupted = nullptr;
-    j->bytes = engine_footprint(j->p.use_double, n, I);
+    // with an edge list, the upper bound of the incidences, plus the
+    // preprocessing's own buffers (endpoints, weights, two key/value pairs
+    // per edge, twice for the sort)
+    j->bytes = engine_footprint(j->p.use_double, n, I) + (e ? (size_t) e->m * 48 : 0);
     std::lock_guard<std::mutex> lk(s->mu);
     j->id = (int) s->jobs.size();
     j->t_submit = now_s();
@@ -364,6 +391,21 @@ extern "C" int fa2_serve_submit(fa2_server *s, const fa2_full_params *p, const d
     s->jobs.push_back(std::move(j));
     s->cv_work.notify_all();
     return s->jobs.back()->id;
+}
+
+extern "C" int fa2_serve_submit(fa2_server *s, const fa2_full_params *p, const double *pos)
+{
+    return submit(s, p, nullptr, pos);
+}
+
+extern "C" int fa2_serve_submit_edges(fa2_server *s, const fa2_full_params *p,
+                                      const fa2_edges *edges, const double *pos)
+{
+    if (!edges) {
+        set_message("no edge list");
+        return -1;
+    }
+    return submit(s, p, edges, pos);
 }
 
 extern "C" int fa2_serve_status(fa2_server *s, int job)

@@ -56,11 +56,36 @@ void fa2_tree_build(fa2_tree *t, int n, const double *x, const double *y, const 
 
 typedef struct fa2_batch fa2_batch;
 
+/* A graph given by its edge list, for the preprocessing on the device:
+ * m edges from[e] -- to[e] (0-based), weights w (already transformed as by
+ * R/forceatlas2.R:254-264; NULL for all 1), and dissuade_hubs. */
+typedef struct {
+    int m;
+    const int *from, *to;
+    const double *w;
+    int outbound;
+} fa2_edges;
+
 /* Upload ngraphs graphs (arguments as for fa2_full_run; the device, the
  * precision and `profile` are those of params[0]) with their initial
  * positions pos[g] (x then y). params[g].iterations caps the iterations of
  * graph g: once reached, the graph stops moving. NULL on error. */
 fa2_batch *fa2_batch_open(int ngraphs, const fa2_full_params *params, const double *const *pos);
+
+/* As fa2_batch_open, with the preprocessing of C_forceatlas2_large
+ * (src/forceatlas2.c:866-902: mass = 1 + degree, incidence lists in edge
+ * order without self-loops, attraction coefficients) done on the device
+ * from edges[g]: the mass, inc_off, inc_node and inc_coef fields of
+ * params[g] are not read. The result is bit-identical to preparing the
+ * graphs on the host. NULL on error. */
+fa2_batch *fa2_batch_open_edges(int ngraphs, const fa2_full_params *params,
+                                const fa2_edges *edges, const double *const *pos);
+
+/* The incidence lists of graph g as the device holds them, in the graph's
+ * own numbering: mass (n), inc_off (n + 1), inc_node (inc_off[n]) and
+ * inc_coef (inc_off[n], in the batch's precision). Any pointer may be NULL. */
+int fa2_batch_csr(fa2_batch *b, int g, double *mass, int *inc_off, int *inc_node,
+                  double *inc_coef);
 
 /* Enqueue `iterations` iterations of every graph; returns without waiting. */
 int fa2_batch_run(fa2_batch *b, int iterations);
@@ -98,8 +123,10 @@ int fa2_batch_stage_times(fa2_batch *b, double *times);
 
 /* Work of the Barnes-Hut repulsion on the tree of the last iteration,
  * counted by an untimed instrumented traversal: out = (regions accepted as
- * monopoles, regions opened, node pairs). */
-int fa2_batch_count_work(fa2_batch *b, unsigned long long out[3]);
+ * monopoles, regions opened, node pairs, steps of the waves' walks). A
+ * single node met as a region's child counts as an accepted region when
+ * its monopole is taken (it is the same term as the pair). */
+int fa2_batch_count_work(fa2_batch *b, unsigned long long out[4]);
 
 /* Replace the state of graph g between two runs: positions (x then y), the
  * force of the previous iteration (x then y; NULL for zero, as at the
@@ -131,6 +158,12 @@ fa2_server *fa2_serve_open(int device, double budget_bytes, int max_batches, int
  * initial positions pos (x then y). The inputs are copied. Returns the
  * request id, or -1. */
 int fa2_serve_submit(fa2_server *s, const fa2_full_params *p, const double *pos);
+
+/* As fa2_serve_submit, with the graph given by its edge list and its
+ * mass and incidence lists prepared on the device (as fa2_batch_open_edges;
+ * p's mass and inc_* fields are not read). */
+int fa2_serve_submit_edges(fa2_server *s, const fa2_full_params *p, const fa2_edges *edges,
+                           const double *pos);
 
 /* 0 queued, 1 running, 2 done, 3 failed; -1 for an unknown request. */
 int fa2_serve_status(fa2_server *s, int job);
