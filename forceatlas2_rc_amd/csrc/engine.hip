@@ -470,12 +470,8 @@ __device__ inline Summ<R> leaf_summ(const typename Vec<R>::v4 p)
     return Summ<R>{p.x, p.y, p.x, p.y, p.z, p.x * p.z, p.y * p.z};
 }
 
-// Summaries of region t from those of its two children, merged left then
-// right (pipeline.cl:276-305), and its traversal record (pipeline.cl:312-323)
-// with its skip link. The centre of mass is kept inside the bounding box of
-// the region, as exact arithmetic guarantees: rounded, it can fall an ulp
-// away from a cluster of coincident nodes, which would then see their own
-// region as a far monopole.
+// Summaries of a region from those of its two children, merged left then
+// right (pipeline.cl:276-305).
 template <typename R> __device__ inline Summ<R> merge_summ(const Summ<R> &a, const Summ<R> &b)
 {
     Summ<R> r;
@@ -489,8 +485,10 @@ template <typename R> __device__ inline Summ<R> merge_summ(const Summ<R> &a, con
     return r;
 }
 
-// (cx, cy, M, size^2) of a region (pipeline.cl:312-323), the centre clamped
-// into the bounding box.
+// (cx, cy, M, size^2) of a region (pipeline.cl:312-323). The centre of mass
+// is kept inside the bounding box of the region, as exact arithmetic
+// guarantees: rounded, it can fall an ulp away from a cluster of coincident
+// nodes, which would then see their own region as a far monopole.
 template <typename R> __device__ inline typename Vec<R>::v4 region_record(const Summ<R> &r)
 {
     const R w = r.x1 - r.x0, h = r.y1 - r.y0;
@@ -499,33 +497,47 @@ template <typename R> __device__ inline typename Vec<R>::v4 region_record(const 
     return mk4<R>(cx, cy, r.m, (R) 2 * (w * w + h * h));
 }
 
+// One level of the climbs of the second and third levels: the summaries of
+// region t from those of its two children, merged left then right
+// (pipeline.cl:276-305), its traversal record (pipeline.cl:312-323) with its
+// skip link, and its parent returned with the parent's range.
+// Every load that depends only on t is issued first, and the loads that
+// depend on those together, before any store: a level costs two round trips
+// to memory instead of one per load.
 template <typename R>
-__device__ inline void merge_region(int t, int g, int gend, int NI,
-                                    const typename Vec<R>::v4 *__restrict__ psort,
-                                    const int2 *__restrict__ child,
-                                    const int *__restrict__ escape,
-                                    typename Vec<R>::v4 *nd, TNode<R> *tn)
+__device__ inline int merge_and_parent(int t, int g, int gend, int NI,
+                                       const typename Vec<R>::v4 *__restrict__ psort,
+                                       const int2 *__restrict__ child,
+                                       const int *__restrict__ parent_int,
+                                       const int *__restrict__ escape,
+                                       typename Vec<R>::v4 *__restrict__ nd,
+                                       TNode<R> *__restrict__ tn, int4 &prange)
 {
+    using V4 = typename Vec<R>::v4;
     const int2 c = child[t];
-    Summ<R> a, b;
-    if (c.x < 0) {
-        a = leaf_summ<R>(psort[~c.x]);
-    } else {
-        const typename Vec<R>::v4 u = nd[2 * c.x], v = nd[2 * c.x + 1];
-        a = Summ<R>{u.x, u.y, u.z, u.w, v.x, v.y, v.z};
+    const int last = tn[t].l.w;
+    const int p = parent_int[t];
+    V4 au, av, bu, bv;
+    if (c.x < 0) au = psort[~c.x];
+    else {
+        au = nd[2 * c.x];
+        av = nd[2 * c.x + 1];
     }
-    if (c.y < 0) {
-        b = leaf_summ<R>(psort[~c.y]);
-    } else {
-        const typename Vec<R>::v4 u = nd[2 * c.y], v = nd[2 * c.y + 1];
-        b = Summ<R>{u.x, u.y, u.z, u.w, v.x, v.y, v.z};
+    if (c.y < 0) bu = psort[~c.y];
+    else {
+        bu = nd[2 * c.y];
+        bv = nd[2 * c.y + 1];
     }
+    const int esc = last == gend - 1 ? 0 : escape[last - g];
+    prange = p >= 0 ? tn[p].l : make_int4(0, 0, 0, 0);
+    const Summ<R> a = c.x < 0 ? leaf_summ<R>(au) : Summ<R>{au.x, au.y, au.z, au.w, av.x, av.y, av.z};
+    const Summ<R> b = c.y < 0 ? leaf_summ<R>(bu) : Summ<R>{bu.x, bu.y, bu.z, bu.w, bv.x, bv.y, bv.z};
     const Summ<R> r = merge_summ<R>(a, b);
     nd[2 * t] = mk4<R>(r.x0, r.y0, r.x1, r.y1);
     nd[2 * t + 1] = mk4<R>(r.m, r.mx, r.my, (R) 0);
     tn[t].c = region_record<R>(r);
-    const int last = tn[t].l.w;
-    tn[t].l.y = last == gend - 1 ? kNoRecord : record_of(escape[last - g], NI);
+    tn[t].l.y = last == gend - 1 ? kNoRecord : record_of(esc, NI);
+    return p;
 }
 
 // Leaf to root in three levels, the second child to arrive at a region
@@ -705,10 +717,10 @@ k_aggregate_super(const Super *__restrict__ supers, const GraphDesc *__restrict_
     for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
         int t = ready[S.lo + q];
         for (int depth = 0; depth < 128; depth++) {
-            merge_region<R>(t, g, gend, NI, psort, child, escape, nd, tn);
-            const int p = parent_int[t];
+            int4 l;
+            const int p = merge_and_parent<R>(t, g, gend, NI, psort, child, parent_int, escape,
+                                              nd, tn, l);
             if (p < 0) break;
-            const int4 l = tn[p].l;
             if (l.z < S.lo || l.w >= S.hi) { // spans several super-tiles
                 if (atomicAdd(&gvis[p], 1) == 1) push_ready(D, p, l.z, l.w, NS, N, ready, rcnt);
                 break;
@@ -735,8 +747,9 @@ k_aggregate_top(const GraphDesc *__restrict__ gd, const typename Vec<R>::v4 *__r
         const int g = gi[t];
         const int gend = gd[g].noff + gd[g].n;
         for (int depth = 0; depth < 128; depth++) {
-            merge_region<R>(t, g, gend, NI, psort, child, escape, nd, tn);
-            const int p = parent_int[t];
+            int4 l;
+            const int p = merge_and_parent<R>(t, g, gend, NI, psort, child, parent_int, escape,
+                                              nd, tn, l);
             if (p < 0) break;
             const int old = __hip_atomic_fetch_add(&gvis[p], 1, __ATOMIC_ACQ_REL,
                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -828,7 +841,6 @@ k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restric
         const bool mine = my_g == g;
         todo &= ~__ballot(mine);
         const GraphDesc G = gd[g];
-        const int gend = G.noff + G.n;
         const bool adjust = ANY_ADJUST && (G.flags & GF_ADJUST);
         const R theta2 = (R) G.theta2;
         int done = mine ? INT_MIN + 1 : INT_MAX; // lanes of other graphs never take part
