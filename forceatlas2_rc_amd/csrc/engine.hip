@@ -573,7 +573,8 @@ __device__ inline void push_ready(const GraphDesc &D, int t, int first, int last
 }
 
 // Level 1 in LDS. The workgroup of a block of sorted positions [lo, hi)
-// (a tile in single precision, half a tile in double, for the LDS budget)
+// (half a tile in single precision, a quarter in double: 60 and 52 KB of LDS,
+// two workgroups per CU; whole tiles measured slower, at one per CU)
 // first stages the block's leaf records and the links of the regions that
 // may lie inside it (internal ids tbase .. tbase + (hi - lo) - 1); a region
 // that leaves the block gets an arrival count that starts at kOutside, so
@@ -583,7 +584,7 @@ __device__ inline void push_ready(const GraphDesc &D, int t, int first, int last
 // whose parent leaves the block (read by the next levels).
 constexpr int kOutside = 1 << 24;
 
-template <typename R> constexpr int agg_block() { return sizeof(R) == 4 ? kTile : kTile / 2; }
+template <typename R> constexpr int agg_block() { return sizeof(R) == 4 ? kTile / 2 : kTile / 4; }
 
 template <typename R>
 __global__ void __launch_bounds__(1024)
