@@ -199,6 +199,7 @@ k_morton(const Tile *__restrict__ tiles, const typename Vec<R>::v2 *__restrict__
 
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
+static_assert(kSortThreads == 256, "the sort kernels hold one digit per thread, four waves");
 constexpr int kTile = kSortThreads * kSortItems; // 2048 nodes
 
 __global__ void __launch_bounds__(kSortThreads)
@@ -332,35 +333,38 @@ k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     }
     __syncthreads();
 
-    // per digit: counts of the waves, tile total, then the tile's digit starts
+    // per digit: counts of the waves, tile total, then the tile's digit
+    // starts; and the digit bases of the graph from its 256 row totals. Both
+    // are exclusive scans over the 256 digits, one digit per thread: within
+    // each wave by shuffles, then across the four waves.
     const uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
     const uint32_t tot = c0 + c1 + c2 + c3;
-    // block exclusive scan of tot over the 256 digits
-    scan_tmp[tid] = tot;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-        const uint32_t u = tid >= o ? scan_tmp[tid - o] : 0u;
-        __syncthreads();
-        scan_tmp[tid] += u;
-        __syncthreads();
+    const uint32_t rt = rowtot[t.g * 256 + tid];
+    uint32_t a = tot, b = rt; // inclusive scans within the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t ua = __shfl_up(a, o, 64), ub = __shfl_up(b, o, 64);
+        if (lane >= o) {
+            a += ua;
+            b += ub;
+        }
     }
-    const uint32_t start = scan_tmp[tid] - tot;
+    if (lane == 63) {
+        scan_tmp[w] = a;
+        scan_tmp[4 + w] = b;
+    }
+    __syncthreads();
+    uint32_t pa = 0, pb = 0;
+    for (int v = 0; v < w; v++) {
+        pa += scan_tmp[v];
+        pb += scan_tmp[4 + v];
+    }
+    const uint32_t start = pa + a - tot;
+    const uint32_t gbase = pb + b - rt;
     wcnt[0][tid] = start;
     wcnt[1][tid] = start + c0;
     wcnt[2][tid] = start + c0 + c1;
     wcnt[3][tid] = start + c0 + c1 + c2;
-    // digit bases of the graph: exclusive scan of its 256 row totals
-    const uint32_t rt = rowtot[t.g * 256 + tid];
-    __syncthreads();
-    scan_tmp[tid] = rt;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-        const uint32_t u = tid >= o ? scan_tmp[tid - o] : 0u;
-        __syncthreads();
-        scan_tmp[tid] += u;
-        __syncthreads();
-    }
-    const uint32_t gbase = scan_tmp[tid] - rt;
     const uint32_t pre = hist[(size_t) G.toff * 256 + (size_t) tid * G.ntiles + t.j];
     goff[tid] = (int) (G.noff + gbase + pre) - (int) start;
     __syncthreads();
