@@ -14,6 +14,9 @@ kernel (timed with HIP events on the batch's stream over the timed region)
 and the CPU baseline (the C restatement of the reference CPU layout,
 Gephi Barnes-Hut in FP64 with OpenMP, on a bounded sample of iterations of
 the same graph). Beside it, in the same line:
+  * "allpairs": the exact all-pairs repulsion kernel of the offload path on
+    a 50,000-node preferential-attachment graph, FP32, its FLOP/s against
+    the FP32 peak (12 FLOP per ordered pair);
   * "batched": 512 stochastic-block-model graphs (200..20000 nodes,
     log-uniform; 2..20 equal blocks; mean in-block degree 8, cross-block 1),
     200 iterations each, laid out as one batch and one at a time;
@@ -83,6 +86,49 @@ def cpu_baseline(n, e, pos, args):
                         f"same {n}-node graph, Gephi Barnes-Hut theta=1.2, FP64, "
                         f"{threads} OpenMP threads, {dt:.1f} s"),
                 prepare_s=prep)
+
+
+def allpairs_leg(args):
+    """The all-pairs repulsion of the offload path (fa2_gpu_repulse_exact) on
+    a 50,000-node preferential-attachment graph (m = 3), FP32: the kernel's
+    time from HIP events on the session's stream, 12 FLOP per ordered pair."""
+    import ctypes
+    from forceatlas2_rc_amd import graphs
+    from forceatlas2_rc_amd._lib import last_error, lib
+    n, e = graphs.barabasi_albert(args.allpairs_nodes, 3, seed=args.seed + 6)
+    mass = 1.0 + np.bincount(e.ravel(), minlength=n).astype(np.float64)
+    rng = np.random.default_rng(args.seed + 7)
+    r = 10.0 * np.sqrt(n)
+    x = rng.uniform(-r, r, n)
+    y = rng.uniform(-r, r, n)
+    dx = np.zeros(n)
+    dy = np.zeros(n)
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    g = lib.fa2_gpu_open(0, 0, n, 0, 0, 1)
+    if not g:
+        raise RuntimeError(last_error())
+    ms = []
+    try:
+        for it in range(args.allpairs_calls + 2):
+            if lib.fa2_gpu_repulse_exact(g, ptr(x), ptr(y), ptr(mass), None, 2.0, ptr(dx),
+                                         ptr(dy)) != 0:
+                raise RuntimeError(last_error())
+            t = ctypes.c_double()
+            if lib.fa2_gpu_kernel_time(g, ctypes.byref(t)) != 0:
+                raise RuntimeError(last_error())
+            if it >= 2:
+                ms.append(t.value)
+    finally:
+        lib.fa2_gpu_close(g)
+    k_ms = float(np.mean(ms))
+    flop = 12.0 * n * (n - 1)
+    achieved = flop / (k_ms * 1e-3) / 1e12
+    log(f"all-pairs: {n} nodes, {k_ms:.3f} ms per kernel, {achieved:.1f} TFLOP/s")
+    return dict(workload=(f"all-pairs repulsion (fa2_gpu_repulse_exact), preferential attachment "
+                          f"n={n}, m=3, FP32"), nodes=n, pairs=n * (n - 1), calls=len(ms),
+                kernel="k_exact", kernel_ms=k_ms, flop_per_launch=flop, achieved=achieved,
+                peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP32_PEAK_TFLOPS,
+                bound="valu")
 
 
 def batched_leg(args):
@@ -249,6 +295,9 @@ def main():
     ap.add_argument("--precision", default="single")
     ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-allpairs", action="store_true")
+    ap.add_argument("--allpairs-nodes", type=int, default=50_000)
+    ap.add_argument("--allpairs-calls", type=int, default=10)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--batch-iters", type=int, default=200)
@@ -363,6 +412,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(n, e, pos, args)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     del graph, mass, off, node, coef
+    if not args.no_allpairs:
+        out["allpairs"] = allpairs_leg(args)
     if not args.no_batched:
         out["batched"] = batched_leg(args)
     if not args.no_serving:

@@ -32,7 +32,7 @@ using namespace fa2;
 namespace {
 
 constexpr int kExactThreads = 256;
-constexpr int kExactTargets = 2;    // targets per thread
+constexpr int kExactTargets = 4;    // targets per thread
 constexpr int kExactTile = 512;     // sources per LDS tile
 constexpr int kBhThreads = 256;
 
@@ -56,17 +56,23 @@ __global__ void k_pack_nodes(int n, const double *__restrict__ x, const double *
     p[i] = v;
 }
 
-// Exact repulsion of targets [blockIdx.x*512, +512) from the sources of
-// slice blockIdx.y; partial[slice][i] receives the sum (x, y).
+// Exact repulsion of targets [blockIdx.x * 1024, +1024) from the sources of
+// slice blockIdx.y; partial[slice][i] receives the sum over the slice of
+// m_j * k(d^2) * (x_i - x_j, y_i - y_j), the factor scaling * m_i of every
+// term being applied once per target by k_sum_slices. Without sizes the
+// weight is m_j / max(d^2, 1e-30): a source at the target's own position has
+// a zero offset, and any two distinct float32 positions are so far apart
+// that d^2 is far above 1e-30, so this equals the reference's factor
+// (0 for d^2 = 0, else scaling m_i m_j / d^2) term by term.
 template <typename R, bool ADJUST>
 __global__ void __launch_bounds__(kExactThreads)
-k_exact(int n, int slice_len, R scaling, const typename Vec<R>::v4 *__restrict__ p,
+k_exact(int n, int slice_len, const typename Vec<R>::v4 *__restrict__ p,
         typename Vec<R>::v2 *__restrict__ partial)
 {
     using V4 = typename Vec<R>::v4;
     __shared__ V4 tile[kExactTile];
     const int i0 = blockIdx.x * (kExactThreads * kExactTargets) + threadIdx.x;
-    R xi[kExactTargets], yi[kExactTargets], mi[kExactTargets], si[kExactTargets];
+    R xi[kExactTargets], yi[kExactTargets], si[kExactTargets];
     R ax[kExactTargets], ay[kExactTargets];
 #pragma unroll
     for (int t = 0; t < kExactTargets; t++) {
@@ -74,7 +80,6 @@ k_exact(int n, int slice_len, R scaling, const typename Vec<R>::v4 *__restrict__
         V4 v = i < n ? p[i] : V4{(R) 0, (R) 0, (R) 0, (R) 0};
         xi[t] = v.x;
         yi[t] = v.y;
-        mi[t] = scaling * v.z; // scaling * m_i, multiplied by m_j per pair
         si[t] = v.w;
         ax[t] = (R) 0;
         ay[t] = (R) 0;
@@ -88,15 +93,19 @@ k_exact(int n, int slice_len, R scaling, const typename Vec<R>::v4 *__restrict__
             tile[t] = t < cnt ? p[j0 + t] : V4{(R) 0, (R) 0, (R) 0, (R) 0};
         __syncthreads();
         // Padding sources have mass 0: their contribution is exactly 0.
-#pragma unroll 4
+#pragma unroll 2
         for (int jj = 0; jj < kExactTile; jj++) {
             const V4 s = tile[jj];
 #pragma unroll
             for (int t = 0; t < kExactTargets; t++) {
                 const R xd = xi[t] - s.x, yd = yi[t] - s.y;
-                const R d2 = xd * xd + yd * yd;
-                const R f = ADJUST ? pair_factor_adjust<R>(xd, yd, mi[t] * s.z, si[t], s.w)
-                                   : pair_factor<R>(d2, mi[t] * s.z);
+                R f;
+                if (ADJUST) {
+                    f = pair_factor_adjust<R>(xd, yd, s.z, si[t], s.w);
+                } else {
+                    const R d2 = xd * xd + yd * yd;
+                    f = s.z * frcp_r(max(d2, (R) 1e-30));
+                }
                 ax[t] += xd * f;
                 ay[t] += yd * f;
             }
@@ -113,6 +122,7 @@ k_exact(int n, int slice_len, R scaling, const typename Vec<R>::v4 *__restrict__
 // f[i] = sum over slices, in slice order, in FP64.
 template <typename R>
 __global__ void k_sum_slices(int n, int slices, const typename Vec<R>::v2 *__restrict__ partial,
+                             const double *__restrict__ mass, double scaling,
                              double2 *__restrict__ f)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -123,7 +133,8 @@ __global__ void k_sum_slices(int n, int slices, const typename Vec<R>::v2 *__res
         sx += (double) v.x;
         sy += (double) v.y;
     }
-    f[i] = make_double2(sx, sy);
+    const double sm = scaling * mass[i]; // the factor scaling * m_i of every term
+    f[i] = make_double2(sx * sm, sy * sm);
 }
 
 // Regions of the host tree: rr = (cx, cy, mass, size^2), ri = (start, count,
@@ -201,6 +212,8 @@ k_bh_host_tree(int n, R scaling, R theta2, const typename Vec<R>::v4 *__restrict
 struct fa2_gpu {
     int device, n, use_double, adjust, barnes_hut, threads;
     hipStream_t stream;
+    hipEvent_t ev[2] = {nullptr, nullptr}; // around the last repulsion kernel
+    int timed = 0;
     double t_send, t_compute, t_fetch;
     int slices, slice_len;
     DevBuf<double> d_in;       // x, y, mass, size (4n) or px, py, pm, ps
@@ -221,7 +234,25 @@ extern "C" void fa2_gpu_close(fa2_gpu *g)
 {
     if (!g) return;
     if (g->stream) (void) hipStreamSynchronize(g->stream);
+    for (hipEvent_t e : g->ev)
+        if (e) (void) hipEventDestroy(e);
     delete g;
+}
+
+extern "C" int fa2_gpu_kernel_time(const fa2_gpu *g, double *ms)
+{
+    if (!g || !g->timed) {
+        set_message("no repulsion kernel has run in this session");
+        return 1;
+    }
+    float t = 0.0f;
+    const hipError_t e = hipEventElapsedTime(&t, g->ev[0], g->ev[1]);
+    if (e != hipSuccess) {
+        set_hip_error("could not read the kernel time", e);
+        return 1;
+    }
+    *ms = (double) t;
+    return 0;
 }
 
 extern "C" void fa2_gpu_times(const fa2_gpu *g, double times[3])
@@ -307,14 +338,21 @@ static hipError_t launch_exact(fa2_gpu *g, double scaling)
         n, in, in + n, in + 2 * (size_t) n, g->adjust ? in + 3 * (size_t) n : nullptr,
         (V4 *) g->d_p.p);
     dim3 grid(ceil_div(n, kExactThreads * kExactTargets), g->slices);
+    if (!g->ev[0]) {
+        (void) hipEventCreate(&g->ev[0]);
+        (void) hipEventCreate(&g->ev[1]);
+    }
+    (void) hipEventRecord(g->ev[0], g->stream);
     if (g->adjust)
         k_exact<R, true><<<grid, kExactThreads, 0, g->stream>>>(
-            n, g->slice_len, (R) scaling, (const V4 *) g->d_p.p, (V2 *) g->d_partial.p);
+            n, g->slice_len, (const V4 *) g->d_p.p, (V2 *) g->d_partial.p);
     else
         k_exact<R, false><<<grid, kExactThreads, 0, g->stream>>>(
-            n, g->slice_len, (R) scaling, (const V4 *) g->d_p.p, (V2 *) g->d_partial.p);
+            n, g->slice_len, (const V4 *) g->d_p.p, (V2 *) g->d_partial.p);
+    (void) hipEventRecord(g->ev[1], g->stream);
+    g->timed = 1;
     k_sum_slices<R><<<ceil_div(n, 256), 256, 0, g->stream>>>(
-        n, g->slices, (const V2 *) g->d_partial.p, g->d_f64.p);
+        n, g->slices, (const V2 *) g->d_partial.p, in + 2 * (size_t) n, scaling, g->d_f64.p);
     return hipGetLastError();
 }
 
@@ -378,6 +416,11 @@ static hipError_t launch_bh(fa2_gpu *g, int nreg, double scaling, double theta)
                              in + 2 * (size_t) n, g->adjust ? in + 3 * (size_t) n : nullptr,
                              (const double4 *) g->d_rr.p, g->d_ri.p, (double2 *) g->d_f.p);
     const R th2 = (R) (theta * theta);
+    if (!g->ev[0]) {
+        (void) hipEventCreate(&g->ev[0]);
+        (void) hipEventCreate(&g->ev[1]);
+    }
+    (void) hipEventRecord(g->ev[0], g->stream);
     if (g->adjust)
         k_bh_host_tree<R, true><<<ceil_div(n, kBhThreads), kBhThreads, 0, g->stream>>>(
             n, (R) scaling, th2, (const V4 *) g->d_p.p, (const V4 *) g->d_rr.p, g->d_ri.p,
@@ -386,6 +429,8 @@ static hipError_t launch_bh(fa2_gpu *g, int nreg, double scaling, double theta)
         k_bh_host_tree<R, false><<<ceil_div(n, kBhThreads), kBhThreads, 0, g->stream>>>(
             n, (R) scaling, th2, (const V4 *) g->d_p.p, (const V4 *) g->d_rr.p, g->d_ri.p,
             (V2 *) g->d_f.p);
+    (void) hipEventRecord(g->ev[1], g->stream);
+    g->timed = 1;
     return hipGetLastError();
 }
 

@@ -52,6 +52,14 @@ int fa2_layout_large_run(int n, int m, const int *from, const int *to, const dou
 void fa2_tree_build(fa2_tree *t, int n, const double *x, const double *y, const double *mass,
                     const double *size, int threads);
 
+/* Device time in milliseconds of the last repulsion kernel of a
+ * fa2_gpu_open session (the all-pairs or the Barnes-Hut kernel; the packing
+ * and summing kernels around it excluded), measured with HIP events on the
+ * session's stream. 0 on success, 1 before any repulsion call. No
+ * counterpart in the reference, whose fa2_gpu_times (src/gpu.c:386-391)
+ * returns host wall times. */
+int fa2_gpu_kernel_time(const fa2_gpu *g, double *ms);
+
 /* ----- batches ---------------------------------------------------------- */
 
 typedef struct fa2_batch fa2_batch;
