@@ -45,6 +45,9 @@ __device__ inline double frcp_r(double x) { return 1.0 / x; }
 __device__ inline float fsqrt_r(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ inline double fsqrt_r(double x) { return __builtin_sqrt(x); }
 
+__device__ inline float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 __device__ inline float flog1p_r(float x) { return log1pf(x); }
 __device__ inline double flog1p_r(double x) { return log1p(x); }
 

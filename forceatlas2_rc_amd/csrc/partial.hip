@@ -60,10 +60,11 @@ __global__ void k_pack_nodes(int n, const double *__restrict__ x, const double *
 // slice blockIdx.y; partial[slice][i] receives the sum over the slice of
 // m_j * k(d^2) * (x_i - x_j, y_i - y_j), the factor scaling * m_i of every
 // term being applied once per target by k_sum_slices. Without sizes the
-// weight is m_j / max(d^2, 1e-30): a source at the target's own position has
-// a zero offset, and any two distinct float32 positions are so far apart
-// that d^2 is far above 1e-30, so this equals the reference's factor
-// (0 for d^2 = 0, else scaling m_i m_j / d^2) term by term.
+// weight is m_j / (d^2 + 1e-30): a source at the target's own position has a
+// zero offset (its finite weight times 0 adds 0), and any two distinct
+// float32 positions are so far apart that adding 1e-30 leaves their d^2
+// unchanged, so this equals the reference's factor (0 for d^2 = 0, else
+// scaling m_i m_j / d^2) term by term.
 template <typename R, bool ADJUST>
 __global__ void __launch_bounds__(kExactThreads)
 k_exact(int n, int slice_len, const typename Vec<R>::v4 *__restrict__ p,
@@ -103,8 +104,9 @@ k_exact(int n, int slice_len, const typename Vec<R>::v4 *__restrict__ p,
                 if (ADJUST) {
                     f = pair_factor_adjust<R>(xd, yd, s.z, si[t], s.w);
                 } else {
-                    const R d2 = xd * xd + yd * yd;
-                    f = s.z * frcp_r(max(d2, (R) 1e-30));
+                    // d^2 + 1e-30 in two fused multiply-adds
+                    const R d2 = fma_r(xd, xd, fma_r(yd, yd, (R) 1e-30));
+                    f = s.z * frcp_r(d2);
                 }
                 ax[t] += xd * f;
                 ay[t] += yd * f;
@@ -331,17 +333,23 @@ static hipError_t launch_exact(fa2_gpu *g, double scaling)
     using V2 = typename Vec<R>::v2;
     const int n = g->n;
     const double *in = g->d_in.p;
-    if (std::is_same<R, double>::value) // the reference's operations, in order
-        return launch_exact_f64(g->stream, n, scaling, g->adjust, in, in + n, in + 2 * (size_t) n,
-                                g->adjust ? in + 3 * (size_t) n : nullptr, g->d_f64.p);
-    k_pack_nodes<R><<<ceil_div(n, 256), 256, 0, g->stream>>>(
-        n, in, in + n, in + 2 * (size_t) n, g->adjust ? in + 3 * (size_t) n : nullptr,
-        (V4 *) g->d_p.p);
-    dim3 grid(ceil_div(n, kExactThreads * kExactTargets), g->slices);
     if (!g->ev[0]) {
         (void) hipEventCreate(&g->ev[0]);
         (void) hipEventCreate(&g->ev[1]);
     }
+    if (std::is_same<R, double>::value) { // the reference's operations, in order
+        (void) hipEventRecord(g->ev[0], g->stream);
+        const hipError_t e =
+            launch_exact_f64(g->stream, n, scaling, g->adjust, in, in + n, in + 2 * (size_t) n,
+                             g->adjust ? in + 3 * (size_t) n : nullptr, g->d_f64.p);
+        (void) hipEventRecord(g->ev[1], g->stream);
+        g->timed = 1;
+        return e;
+    }
+    k_pack_nodes<R><<<ceil_div(n, 256), 256, 0, g->stream>>>(
+        n, in, in + n, in + 2 * (size_t) n, g->adjust ? in + 3 * (size_t) n : nullptr,
+        (V4 *) g->d_p.p);
+    dim3 grid(ceil_div(n, kExactThreads * kExactTargets), g->slices);
     (void) hipEventRecord(g->ev[0], g->stream);
     if (g->adjust)
         k_exact<R, true><<<grid, kExactThreads, 0, g->stream>>>(
@@ -411,16 +419,21 @@ static hipError_t launch_bh(fa2_gpu *g, int nreg, double scaling, double theta)
         (V4 *) g->d_p.p);
     k_pack_regions<R><<<ceil_div(nreg, 256), 256, 0, g->stream>>>(
         nreg, g->d_reg.p, (V4 *) g->d_rr.p, g->d_ri.p);
-    if (std::is_same<R, double>::value) // the reference's operations, in order
-        return launch_bh_f64(g->stream, n, scaling, theta * theta, g->adjust, in, in + n,
-                             in + 2 * (size_t) n, g->adjust ? in + 3 * (size_t) n : nullptr,
-                             (const double4 *) g->d_rr.p, g->d_ri.p, (double2 *) g->d_f.p);
-    const R th2 = (R) (theta * theta);
     if (!g->ev[0]) {
         (void) hipEventCreate(&g->ev[0]);
         (void) hipEventCreate(&g->ev[1]);
     }
     (void) hipEventRecord(g->ev[0], g->stream);
+    if (std::is_same<R, double>::value) { // the reference's operations, in order
+        const hipError_t e =
+            launch_bh_f64(g->stream, n, scaling, theta * theta, g->adjust, in, in + n,
+                          in + 2 * (size_t) n, g->adjust ? in + 3 * (size_t) n : nullptr,
+                          (const double4 *) g->d_rr.p, g->d_ri.p, (double2 *) g->d_f.p);
+        (void) hipEventRecord(g->ev[1], g->stream);
+        g->timed = 1;
+        return e;
+    }
+    const R th2 = (R) (theta * theta);
     if (g->adjust)
         k_bh_host_tree<R, true><<<ceil_div(n, kBhThreads), kBhThreads, 0, g->stream>>>(
             n, (R) scaling, th2, (const V4 *) g->d_p.p, (const V4 *) g->d_rr.p, g->d_ri.p,
