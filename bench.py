@@ -9,9 +9,15 @@ iteration of the whole graph: codes, sort, tree, repulsion, attraction,
 speed, displacement. The graph and its state are resident in HBM before the
 timed region starts.
 
-Printed: one JSON line with node-iters/s, the roofline of the dominant
-kernel (timed with HIP events on the batch's stream over the timed region)
-and the CPU baseline (the C restatement of the reference CPU layout,
+The timed iterations are replayed from the captured HIP graph, as a layout
+runs. The same number of iterations is then launched one kernel at a time
+with HIP events on the batch's stream between the phases: their times give
+the phase breakdown, the roofline of the dominant kernel (the Barnes-Hut
+traversal, FP32 VALU) and the iteration roofline (per phase, the larger of
+minimum HBM bytes over 8 TB/s and FLOP over the FP32 peak, summed, over the
+measured iteration time), with each phase's HBM rate.
+
+Printed: one JSON line with node-iters/s, the rooflines and the CPU baseline (the C restatement of the reference CPU layout,
 Gephi Barnes-Hut in FP64 with OpenMP, on a bounded sample of iterations of
 the same graph). Beside it, in the same line:
   * "allpairs": the exact all-pairs repulsion kernel of the offload path on
@@ -25,8 +31,9 @@ the same graph). Beside it, in the same line:
     100..1000 iterations; linlog, prevent_overlap with node sizes in [1, 3],
     5% pinned nodes and double precision mixed in) served by the layout
     server, against the same requests solved one at a time; p50/p99
-    latency and layouts/s. Both start from prepared incidence lists and
-    include the upload and the download of every graph.
+    latency and layouts/s, at the offered rate and again at 3 requests/s
+    (below the server's capacity). Both start from prepared incidence lists
+    and include the upload and the download of every graph.
 """
 import argparse
 import json
@@ -86,6 +93,35 @@ def cpu_baseline(n, e, pos, args):
                         f"same {n}-node graph, Gephi Barnes-Hut theta=1.2, FP64, "
                         f"{threads} OpenMP threads, {dt:.1f} s"),
                 prepare_s=prep)
+
+
+# Minimum HBM bytes per node of each phase of one FP32 iteration (positions,
+# masses, keys, tree records and forces each read or written once), and per
+# incidence for the attraction's list entry and position gather: 456n + 16I
+# in total. FP64 position accumulators add 16 bytes per node to the move.
+STAGE_BYTES_PER_NODE = dict(codes=8 + 16,            # bbox, Morton codes
+                            sort=4 * 16 + 32,         # 4 LSD passes, payload gather
+                            tree=32 + 144 + 40,       # Karras, summaries, finalize
+                            repulsion=28,             # streaming part of the traversal
+                            attraction=32,            # per node; + 16 per incidence
+                            move=24 + 36)             # swing, apply
+
+
+def iteration_roofline(n, n_inc, rep_flop, stages, t_iter, flop_peak):
+    """t_roof = sum over phases of max(bytes / HBM peak, FLOP / FLOP peak),
+    and the HBM rate each phase achieves on its minimum bytes."""
+    out, t_roof = {}, 0.0
+    for k, per_node in STAGE_BYTES_PER_NODE.items():
+        nbytes = per_node * n + (16 * n_inc if k == "attraction" else 0)
+        flop = rep_flop if k == "repulsion" else 0
+        t_min = max(nbytes / (HBM_PEAK_GBS * 1e9), flop / (flop_peak * 1e12))
+        t_roof += t_min
+        t = stages[k]
+        out[k] = dict(bytes=nbytes, ms=t * 1e3, gbs=nbytes / t / 1e9,
+                      hbm_frac=nbytes / t / 1e9 / HBM_PEAK_GBS, roof_ms=t_min * 1e3)
+    return dict(t_roof_ms=t_roof * 1e3, t_iter_ms=t_iter * 1e3, frac=t_roof / t_iter,
+                bytes_per_iteration=sum(v["bytes"] for v in out.values()),
+                hbm_peak_gbs=HBM_PEAK_GBS, stages=out)
 
 
 def allpairs_leg(args):
@@ -222,19 +258,10 @@ def serving_requests(args):
     return reqs, np.cumsum(gaps)
 
 
-def serving_leg(args):
-    """A Poisson stream of layout requests through the server, against the
-    same requests solved one at a time."""
-    from forceatlas2_rc_amd import Batch, Server
-    t0 = time.time()
-    reqs, arrive = serving_requests(args)
-    n_tot = sum(g["n"] for g, _, _ in reqs)
-    work = sum(g["n"] * g["iterations"] for g, _, _ in reqs)
-    log(f"serving: {len(reqs)} requests, {n_tot} nodes, {work:.3g} node-iterations "
-        f"({time.time() - t0:.1f} s)")
-    with Batch([reqs[0][0]]) as b:  # warm-up
-        b.run(2)
-        b.sync()
+def serve_stream(reqs, arrive, args):
+    """Submit the requests at their arrival times; per-request latency from
+    arrival to result."""
+    from forceatlas2_rc_amd import Server
     with Server(max_batches=args.serve_batches, chunk=args.serve_chunk) as srv:
         ids = []
         t0 = time.perf_counter()
@@ -244,12 +271,42 @@ def serving_leg(args):
                 time.sleep(wait)
             ids.append(srv.submit(g, prec))
         out = [srv.result(j) for j in ids]
-        t_end = time.perf_counter()
+        span = time.perf_counter() - t0
         stats = srv.stats()
     lat = np.array([t["total"] for _, t in out])
-    span = t_end - t0
-    log(f"served: {span:.2f} s, p50 {np.percentile(lat, 50):.2f} s, "
-        f"p99 {np.percentile(lat, 99):.2f} s, {stats}")
+    log(f"served at {len(reqs) / arrive[-1] if arrive[-1] > 0 else 0:.1f} requests/s offered: "
+        f"{span:.2f} s, p50 {np.percentile(lat, 50):.2f} s, p99 {np.percentile(lat, 99):.2f} s, "
+        f"{stats}")
+    return out, span, lat, stats
+
+
+def queue_latency(arrive, svc):
+    """Latencies of the same arrivals served by one solver in turn."""
+    fin, lat = 0.0, []
+    for ta, s_ in zip(arrive, svc):
+        fin = max(fin, ta) + s_
+        lat.append(fin - ta)
+    return np.array(lat)
+
+
+def serving_leg(args):
+    """A Poisson stream of layout requests through the server, against the
+    same requests solved one at a time: at the offered rate (above capacity,
+    so the stream saturates the server) and, with the same requests, at a
+    rate below capacity."""
+    from forceatlas2_rc_amd import Batch
+    t0 = time.time()
+    reqs, arrive = serving_requests(args)
+    n_tot = sum(g["n"] for g, _, _ in reqs)
+    work = sum(g["n"] * g["iterations"] for g, _, _ in reqs)
+    log(f"serving: {len(reqs)} requests, {n_tot} nodes, {work:.3g} node-iterations "
+        f"({time.time() - t0:.1f} s)")
+    with Batch([reqs[0][0]]) as b:  # warm-up
+        b.run(2)
+        b.sync()
+    out, span, lat, stats = serve_stream(reqs, arrive, args)
+    arrive_sub = arrive * (args.serve_rate / args.serve_sub_rate)
+    out_sub, span_sub, lat_sub, stats_sub = serve_stream(reqs, arrive_sub, args)
     # one at a time, in arrival order
     svc, same = [], 0
     for k, (g, prec, _) in enumerate(reqs):
@@ -259,28 +316,31 @@ def serving_leg(args):
             b.sync()
             p = b.positions(0)
         svc.append(time.perf_counter() - t1)
-        same += int(np.array_equal(p, out[k][0]))
+        same += int(np.array_equal(p, out[k][0]) and np.array_equal(p, out_sub[k][0]))
     svc = np.array(svc)
-    # the same arrivals served by one solver in turn
-    fin, lat1 = 0.0, []
-    for ta, s_ in zip(arrive, svc):
-        fin = max(fin, ta) + s_
-        lat1.append(fin - ta)
-    lat1 = np.array(lat1)
+    lat1 = queue_latency(arrive, svc)
+    lat1_sub = queue_latency(arrive_sub, svc)
     log(f"one at a time: {svc.sum():.2f} s of service")
+    pct = lambda a, q: float(np.percentile(a, q))
     return dict(workload=(f"Poisson stream, {args.serve_rate:g} requests/s, {len(reqs)} "
                           f"requests; preferential-attachment or SBM graphs of 1000.."
                           f"{args.serve_max_nodes} nodes (log-uniform), mean degree 4..16, "
                           "100..1000 iterations; linlog, prevent_overlap (sizes 1..3), 5% "
                           "pinned, dissuade_hubs and double precision mixed in"),
                 requests=len(reqs), nodes=n_tot, node_iterations=work,
-                layouts_per_s=len(reqs) / span, p50_s=float(np.percentile(lat, 50)),
-                p99_s=float(np.percentile(lat, 99)),
+                layouts_per_s=len(reqs) / span, p50_s=pct(lat, 50), p99_s=pct(lat, 99),
                 one_at_a_time=dict(layouts_per_s=len(reqs) / svc.sum(),
-                                   p50_s=float(np.percentile(lat1, 50)),
-                                   p99_s=float(np.percentile(lat1, 99))),
+                                   p50_s=pct(lat1, 50), p99_s=pct(lat1, 99),
+                                   service_s=float(svc.sum())),
                 speedup=(len(reqs) / span) / (len(reqs) / svc.sum()),
                 batches=stats["batches"], max_batches_running=stats["peak_running"],
+                below_capacity=dict(rate=args.serve_sub_rate, span_s=span_sub,
+                                    layouts_per_s=len(reqs) / span_sub,
+                                    p50_s=pct(lat_sub, 50), p99_s=pct(lat_sub, 99),
+                                    one_at_a_time_p50_s=pct(lat1_sub, 50),
+                                    one_at_a_time_p99_s=pct(lat1_sub, 99),
+                                    batches=stats_sub["batches"],
+                                    max_batches_running=stats_sub["peak_running"]),
                 identical_results=f"{same}/{len(reqs)}")
 
 
@@ -304,6 +364,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-sub-rate", type=float, default=3.0)
     ap.add_argument("--serve-max-nodes", type=int, default=1_000_000)
     ap.add_argument("--serve-batches", type=int, default=4)
     ap.add_argument("--serve-chunk", type=int, default=25)
@@ -352,19 +413,34 @@ def main():
     b0 = Batch([raw], device=0, precision=args.precision, ingest="device")
     t_dev = time.perf_counter() - t0
     b0.close()
+    # the timed region: iterations replayed from the captured HIP graph
     t0 = time.perf_counter()
-    b = Batch([graph], device=0, precision=args.precision, profile=True)
+    b = Batch([graph], device=0, precision=args.precision)
     t_up = time.perf_counter() - t0
     log(f"uploaded prepared lists in {t_up:.2f} s; device preprocessing and upload {t_dev:.2f} s")
     b.run(args.warmup)
     b.sync()
-    base = b.stage_times()
     log("warm-up done")
-
     t0 = time.perf_counter()
     b.run(args.steps)
     b.sync()
     dt = time.perf_counter() - t0
+    pos_replay = b.positions(0)
+    b.close()
+    ms = dt / args.steps * 1e3
+    value = n * args.steps / dt
+    log(f"timed: {args.steps} iterations, {ms:.4f} ms per iteration")
+
+    # the same iterations launched one kernel at a time with HIP events
+    # between the phases: phase times and the traversal's duration
+    b = Batch([graph], device=0, precision=args.precision, profile=True)
+    b.run(args.warmup)
+    b.sync()
+    base = b.stage_times()
+    t0 = time.perf_counter()
+    b.run(args.steps)
+    b.sync()
+    dt_prof = time.perf_counter() - t0
     st = b.stage_times()
     stages = {k: (st[k] - base[k]) / args.steps for k in st if k != "total"}
     log("per-iteration device time by phase (ms):",
@@ -374,14 +450,15 @@ def main():
     finite = bool(np.all(np.isfinite(pos_out)))
     b.close()
 
-    ms = dt / args.steps * 1e3
-    value = n * args.steps / dt
     A, O_, P, steps = work
     flops = 6 * (A + O_) + 7 * A + 12 * P
     t_rep = stages["repulsion"]
     dbl = args.precision == "double"
     peak = FP64_PEAK_TFLOPS if dbl else FP32_PEAK_TFLOPS
     achieved = flops / t_rep / 1e12
+    n_inc = int(off[-1])
+    iter_roof = iteration_roofline(n, n_inc, flops, stages, ms * 1e-3, peak)
+    log("iteration roofline:", json.dumps(iter_roof))
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "repulse_rmat21_traffic.json")
     if os.path.exists(tpath):
@@ -402,11 +479,14 @@ def main():
                                      "undirected edges, all-GPU FA2, BH "
                                      "theta=1.2, linlog, dissuade_hubs"),
                            nodes=n, edges=int(len(e)), precision=args.precision,
+                           timed_path="HIP graph replay (two iterations per graph)",
                            stage_ms={k: v * 1e3 for k, v in stages.items()},
+                           profiled_ms_per_step=dt_prof / args.steps * 1e3,
+                           replay_equals_eager=bool(np.array_equal(pos_replay, pos_out)),
                            positions_finite=finite,
                            ingest=dict(host_numpy_prepare_s=t_prep, upload_prepared_s=t_up,
                                        device_prepare_with_upload_s=t_dev)),
-               roofline=roof)
+               roofline=roof, iteration_roofline=iter_roof)
     if not args.no_cpu:
         log("cpu baseline...")
         out["cpu_baseline"] = cpu_baseline(n, e, pos, args)
