@@ -198,6 +198,7 @@ def batched_leg(args):
     b.sync()
     t_run = time.perf_counter()
     res_b = [b.positions(g) for g in range(len(gs))]
+    shapes = b.tuning  # chosen by the autotuner at the first run (timed above)
     b.close()
     t_batch = time.perf_counter() - t0
     log(f"batched: {t_batch:.2f} s ({(t_run - t_up) / it * 1e3:.3f} ms per iteration)")
@@ -213,10 +214,11 @@ def batched_leg(args):
     return dict(workload=(f"{len(gs)} SBM graphs, 200..20000 nodes log-uniform, 2..20 blocks, "
                           f"in-block degree 8, cross-block 1, {it} iterations each, BH theta=1.2"),
                 graphs=len(gs), nodes=n_tot, edges=m_tot, iterations=it,
-                batched_s=t_batch, batched_ms_per_iteration=(t_run - t_up) / it * 1e3,
+                batched_s=t_batch, open_s=t_up - t0,
+                batched_ms_per_iteration=(t_run - t_up) / it * 1e3,
                 one_at_a_time_s=t_one, layouts_per_s_batched=len(gs) / t_batch,
                 layouts_per_s_one_at_a_time=len(gs) / t_one, speedup=t_one / t_batch,
-                node_iters_per_s_batched=n_tot * it / t_batch,
+                node_iters_per_s_batched=n_tot * it / t_batch, launch_shapes=shapes,
                 identical_results=f"{same}/{len(gs)}")
 
 
@@ -415,9 +417,12 @@ def main():
     b0.close()
     # the timed region: iterations replayed from the captured HIP graph
     t0 = time.perf_counter()
-    b = Batch([graph], device=0, precision=args.precision)
+    b = Batch([graph], device=0, precision=args.precision, autotune=False)
     t_up = time.perf_counter() - t0
     log(f"uploaded prepared lists in {t_up:.2f} s; device preprocessing and upload {t_dev:.2f} s")
+    # launch shapes: the autotuner times every candidate on this graph
+    shapes, shape_ms = b.tune(force=True)
+    log(f"autotuner: {shapes}, candidate ms {np.round(shape_ms, 4).tolist()}")
     b.run(args.warmup)
     b.sync()
     log("warm-up done")
@@ -433,7 +438,8 @@ def main():
 
     # the same iterations launched one kernel at a time with HIP events
     # between the phases: phase times and the traversal's duration
-    b = Batch([graph], device=0, precision=args.precision, profile=True)
+    b = Batch([graph], device=0, precision=args.precision, profile=True, autotune=False)
+    b.set_tuning(**shapes)
     b.run(args.warmup)
     b.sync()
     base = b.stage_times()
@@ -480,6 +486,11 @@ def main():
                                      "theta=1.2, linlog, dissuade_hubs"),
                            nodes=n, edges=int(len(e)), precision=args.precision,
                            timed_path="HIP graph replay (two iterations per graph)",
+                           launch_shapes=dict(chosen=shapes, candidate_ms=dict(
+                               repulse_threads_64_128_256=shape_ms[:3].tolist(),
+                               attract_lane_unroll_1_2_4_8=shape_ms[3:7].tolist(),
+                               attract_16lane_unroll_1_2_4_8=shape_ms[7:11].tolist(),
+                               attract_wave_unroll_1_2_4_8=shape_ms[11:15].tolist())),
                            stage_ms={k: v * 1e3 for k, v in stages.items()},
                            profiled_ms_per_step=dt_prof / args.steps * 1e3,
                            replay_equals_eager=bool(np.array_equal(pos_replay, pos_out)),

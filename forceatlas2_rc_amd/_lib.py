@@ -55,6 +55,14 @@ class Edges(Structure):
                 ("w", POINTER(c_double)), ("outbound", c_int)]
 
 
+class Tuning(Structure):
+    """fa2_tuning (include/fa2_hip.h): launch shapes of a batch."""
+    _fields_ = [("attract_unroll", c_int * 3), ("repulse_threads", c_int)]
+
+
+FA2_TUNE_NTIMES = 15
+
+
 class FullParams(Structure):
     """fa2_full_params (src/fa2.h:97-115)."""
     _fields_ = [("n", c_int), ("inc_off", POINTER(c_int)),
@@ -115,6 +123,9 @@ PROTOTYPES = {
     "fa2_batch_count_work": (c_int, [c_void_p, c_void_p]),
     "fa2_batch_set_state": (c_int, [c_void_p, c_int, c_void_p, c_void_p,
                                     c_double, c_double]),
+    "fa2_batch_tune": (c_int, [c_void_p, c_int, POINTER(Tuning), c_void_p]),
+    "fa2_batch_get_tuning": (c_int, [c_void_p, POINTER(Tuning)]),
+    "fa2_batch_set_tuning": (c_int, [c_void_p, POINTER(Tuning)]),
     "fa2_batch_stream": (c_void_p, [c_void_p]),
     "fa2_batch_close": (None, [c_void_p]),
     "fa2_serve_open": (c_void_p, [c_int, c_double, c_int, c_int]),

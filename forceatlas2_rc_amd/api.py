@@ -17,8 +17,8 @@ from ctypes import POINTER, byref, c_double, c_int, c_uint32, c_void_p
 
 import numpy as np
 
-from ._lib import (FA2_FULL_NTIMES, DevInfo, Edges, FullParams, GPUError,
-                   INTERRUPT_FN, last_error, lib)
+from ._lib import (FA2_FULL_NTIMES, FA2_TUNE_NTIMES, DevInfo, Edges, FullParams, GPUError,
+                   INTERRUPT_FN, Tuning, last_error, lib)
 
 __all__ = ["gpu_devices", "layout_forceatlas2_gpu", "layout_forceatlas2_large",
            "prepare_weights", "prepare_large", "Batch", "GPUError"]
@@ -344,6 +344,11 @@ def _fill_params(p, gr, device, precision, profile, edges=None):
     return [mass, off, node, coef, size, fixed, xy, frm, to, w], xy
 
 
+def _tuning_dict(t):
+    return dict(attract_unroll=tuple(int(t.attract_unroll[c]) for c in range(3)),
+                repulse_threads=int(t.repulse_threads))
+
+
 class Batch:
     """Graphs laid out together on one device (fa2_batch_* of fa2_hip.h).
 
@@ -354,10 +359,16 @@ class Batch:
     and ``profile`` are common to the batch. ``ingest`` chooses where the
     mass and incidence lists are built: "host" (numpy, or the prepared
     mass/inc_off/inc_node/inc_coef of the dicts) or "device"
-    (fa2_batch_open_edges).
+    (fa2_batch_open_edges). ``autotune`` times the candidate launch shapes
+    (fa2_batch_tune) before the first iteration and keeps the fastest; by
+    default on for batches of at least AUTOTUNE_NODES nodes. The shapes never
+    change a result.
     """
 
-    def __init__(self, graphs, device=0, precision="single", profile=False, ingest="host"):
+    AUTOTUNE_NODES = 65536
+
+    def __init__(self, graphs, device=0, precision="single", profile=False, ingest="host",
+                 autotune=None):
         if ingest not in ("host", "device"):
             raise ValueError("ingest must be 'host' or 'device'")
         self._keep = []
@@ -380,9 +391,40 @@ class Batch:
         if not self.h:
             raise GPUError(last_error())
         self.G = G
+        self._autotune = (sum(self.n) >= self.AUTOTUNE_NODES) if autotune is None else autotune
 
     def run(self, iterations):
+        if self._autotune:
+            self._autotune = False
+            self.tune(force=False)
         if lib.fa2_batch_run(self.h, int(iterations)) != 0:
+            raise GPUError(last_error())
+
+    def tune(self, force=True):
+        """Time the candidate launch shapes on the current positions and keep
+        the fastest (fa2_batch_tune). Returns the chosen shapes and the
+        timings in ms (traversal at 64, 128, 256 threads; then unrolls 1, 2,
+        4, 8 of each attraction class), or None for timings when a cached
+        choice was reused."""
+        t = Tuning()
+        times = np.full(FA2_TUNE_NTIMES, -1.0)
+        if lib.fa2_batch_tune(self.h, int(bool(force)), ctypes.byref(t), _ptr(times)) != 0:
+            raise GPUError(last_error())
+        return _tuning_dict(t), (None if times[0] < 0 else times)
+
+    @property
+    def tuning(self):
+        t = Tuning()
+        if lib.fa2_batch_get_tuning(self.h, ctypes.byref(t)) != 0:
+            raise GPUError(last_error())
+        return _tuning_dict(t)
+
+    def set_tuning(self, attract_unroll=(4, 4, 4), repulse_threads=256):
+        t = Tuning()
+        for c in range(3):
+            t.attract_unroll[c] = int(attract_unroll[c])
+        t.repulse_threads = int(repulse_threads)
+        if lib.fa2_batch_set_tuning(self.h, ctypes.byref(t)) != 0:
             raise GPUError(last_error())
 
     def sync(self):

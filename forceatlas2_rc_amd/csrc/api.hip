@@ -142,6 +142,33 @@ extern "C" int fa2_batch_set_state(fa2_batch *b, int g, const double *pos, const
     return b->e->set_state(g, pos, f_old, speed, speed_efficiency);
 }
 
+extern "C" int fa2_batch_tune(fa2_batch *b, int force, fa2_tuning *chosen, double *times)
+{
+    if (!b) {
+        set_message("no batch");
+        return 1;
+    }
+    return b->e->tune(force, chosen, times);
+}
+
+extern "C" int fa2_batch_get_tuning(fa2_batch *b, fa2_tuning *t)
+{
+    if (!b || !t) {
+        set_message("no batch or no tuning");
+        return 1;
+    }
+    return b->e->get_tuning(t);
+}
+
+extern "C" int fa2_batch_set_tuning(fa2_batch *b, const fa2_tuning *t)
+{
+    if (!b || !t) {
+        set_message("no batch or no tuning");
+        return 1;
+    }
+    return b->e->set_tuning(t);
+}
+
 extern "C" void *fa2_batch_stream(fa2_batch *b) { return b ? (void *) b->e->stream : nullptr; }
 
 extern "C" void fa2_batch_close(fa2_batch *b) { delete b; }

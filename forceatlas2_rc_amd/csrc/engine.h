@@ -67,6 +67,11 @@ struct Engine {
     // (x then y, NULL for zero), speed and speed efficiency.
     virtual int set_state(int graph, const double *pos, const double *fold, double speed,
                           double eff) = 0;
+    // Launch shapes (fa2_tuning of fa2_hip.h); tune() times the candidates
+    // on the current positions and keeps the fastest.
+    virtual int tune(int force, fa2_tuning *chosen, double *times) = 0;
+    virtual int get_tuning(fa2_tuning *t) = 0;
+    virtual int set_tuning(const fa2_tuning *t) = 0;
 };
 
 Engine *make_engine(int use_double);

@@ -25,6 +25,9 @@
 // to run, and a graph gives the same result alone and in a batch as long
 // as the tiles of its nodes do not change (they depend only on the graph).
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -964,6 +967,9 @@ k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restric
 //  * more: work items of at most kChunk consecutive incidences, one wave
 //    each; a node cut into several items gets the sum of their partial sums
 //    in item order (k_attract_fix). Nodes without edges are not visited.
+// A lane adds its incidences e0, e0 + step, ... in that order whatever the
+// unroll U (the number of incidences whose loads are in flight at once), so
+// U changes the speed and never the result.
 // ---------------------------------------------------------------------------
 
 constexpr int kChunk = 1024;
@@ -977,36 +983,40 @@ constexpr int kChunk = 1024;
 __device__ inline float linlog_factor(float d)
 {
     const float u = 1.0f + d;
-    if (u == 1.0f) return 1.0f;
-    return (__builtin_amdgcn_logf(u) * 0.693147180559945309f) * __builtin_amdgcn_rcpf(u - 1.0f);
+    const float r = (__builtin_amdgcn_logf(u) * 0.693147180559945309f) *
+                    __builtin_amdgcn_rcpf(u - 1.0f);
+    return u == 1.0f ? 1.0f : r;
 }
 __device__ inline double linlog_factor(double d) { return log1p(d) / d; }
 
 // The attraction of one incidence (pipeline.cl:412-427): weight
 // w = -c, times log1p(d) / d with linlog; with sizes the distance is the gap
 // between the discs, computed in FP64 as for the repulsion, and the edge only
-// pulls while it is positive. The term w * (xd, yd) goes into the FP64 sum.
+// pulls while it is positive. The term w * (xd, yd) goes into the FP64 sum
+// when `on`.
 template <typename R>
 __device__ inline void attract_term(typename Vec<R>::v2 pi, typename Vec<R>::v2 pj, R c, R si,
-                                    R sj, bool linlog, bool adjust, ForceSum<R> &sum)
+                                    R sj, bool linlog, bool adjust, bool on, ForceSum<R> &sum)
 {
     const R xd = pi.x - pj.x, yd = pi.y - pj.y;
     R w = -c;
     if (adjust) {
         const double dx = (double) xd, dy = (double) yd;
         const double gap = __builtin_sqrt(dx * dx + dy * dy) - ((double) si + (double) sj);
-        if (!(gap > 0.0)) return;
+        on = on && gap > 0.0;
         if (linlog) w *= linlog_factor((R) gap);
     } else if (linlog) {
         const R dist = fsqrt_r(xd * xd + yd * yd);
-        if (!(dist > (R) 0)) return;
+        on = on && dist > (R) 0;
         w *= linlog_factor(dist);
     }
-    sum.add(xd, yd, w);
+    if (on) sum.add(xd, yd, w);
 }
 
-// The incidences e0, e0 + step, ... < e1 of one node, in that order; the
-// loads of U incidences are issued before their arithmetic.
+// The incidences e0, e0 + step, ... < e1 of one node, in that order. Each
+// round first loads the list entries of U incidences, then their U
+// positions, then adds their terms; the index of a slot past e1 is clamped
+// to e1 - 1 (a valid entry), so every load is unconditional.
 template <typename R, int U>
 __device__ inline void attract_terms(int e0, int e1, int step, bool linlog, bool adjust,
                                      typename Vec<R>::v2 pi, R si,
@@ -1014,25 +1024,26 @@ __device__ inline void attract_terms(int e0, int e1, int step, bool linlog, bool
                                      const R *__restrict__ size, const int *__restrict__ inc_node,
                                      const R *__restrict__ inc_coef, ForceSum<R> &sum)
 {
+    const int last = e1 - 1;
     for (int e = e0; e < e1; e += U * step) {
         int j[U];
         R c[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int q = e + u * step;
-            j[u] = q < e1 ? inc_node[q] : -1;
-            c[u] = q < e1 ? inc_coef[q] : (R) 0;
+            const int q = min(e + u * step, last);
+            j[u] = inc_node[q];
+            c[u] = inc_coef[q];
         }
         typename Vec<R>::v2 pj[U];
         R sj[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            pj[u] = j[u] >= 0 ? pos[j[u]] : pi;
-            sj[u] = (adjust && j[u] >= 0) ? size[j[u]] : (R) 0;
+            pj[u] = pos[j[u]];
+            sj[u] = adjust ? size[j[u]] : (R) 0;
         }
 #pragma unroll
         for (int u = 0; u < U; u++)
-            if (j[u] >= 0) attract_term<R>(pi, pj[u], c[u], si, sj[u], linlog, adjust, sum);
+            attract_term<R>(pi, pj[u], c[u], si, sj[u], linlog, adjust, e + u * step < e1, sum);
     }
 }
 
@@ -1045,27 +1056,23 @@ template <int GL> __device__ inline void group_sum(double &x, double &y)
     }
 }
 
-template <typename R, int GL>
+// GL lanes per node; the work list holds (node, first incidence, end, graph).
+template <typename R, int GL, int U>
 __global__ void __launch_bounds__(256)
-k_attract(int count, const int *__restrict__ nodes, const int *__restrict__ gid,
-          const GraphDesc *__restrict__ gd, const GraphState *__restrict__ gs,
-          const typename Vec<R>::v2 *__restrict__ pos,
-          const R *__restrict__ size, const int *__restrict__ inc_off,
-          const int *__restrict__ inc_node, const R *__restrict__ inc_coef,
-          double2 *__restrict__ F)
+k_attract(int count, const int4 *__restrict__ work, const GraphDesc *__restrict__ gd,
+          const GraphState *__restrict__ gs, const typename Vec<R>::v2 *__restrict__ pos,
+          const R *__restrict__ size, const int *__restrict__ inc_node,
+          const R *__restrict__ inc_coef, double2 *__restrict__ F)
 {
     const int slot = blockIdx.x * (256 / GL) + (int) threadIdx.x / GL;
     const int sub = (int) threadIdx.x % GL;
-    const int i = slot < count ? nodes[slot] : nodes[0];
-    const bool live = slot < count && !gs[gid[i]].retired;
-    const int flags = gd[gid[i]].flags;
+    const int4 wk = work[min(slot, count - 1)];
+    const int i = wk.x;
+    const int flags = gd[wk.w].flags;
+    const bool live = slot < count && !gs[wk.w].retired;
     ForceSum<R> sum;
-    if (live)
-        // a lane per node: four incidences in flight per lane; 16 lanes per
-        // node already keep enough loads in flight
-        attract_terms<R, GL == 1 ? 4 : 1>(inc_off[i] + sub, inc_off[i + 1], GL, flags & GF_LINLOG,
-                                          flags & GF_ADJUST, pos[i], size[i], pos, size, inc_node,
-                                          inc_coef, sum);
+    attract_terms<R, U>(wk.y + sub, wk.z, GL, flags & GF_LINLOG, flags & GF_ADJUST, pos[i],
+                        size[i], pos, size, inc_node, inc_coef, sum);
     group_sum<GL>(sum.x, sum.y);
     if (!live || sub != 0) return;
     double2 f = F[i];
@@ -1075,7 +1082,7 @@ k_attract(int count, const int *__restrict__ nodes, const int *__restrict__ gid,
 }
 
 // One wave per item (node, first incidence, end, partial slot or -1).
-template <typename R>
+template <typename R, int U>
 __global__ void __launch_bounds__(256)
 k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict__ gid,
                 const GraphDesc *__restrict__ gd, const GraphState *__restrict__ gs,
@@ -1089,15 +1096,13 @@ k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict
     const int lane = lane_id();
     const int4 it = items[w];
     const int i = it.x;
-    if (gs[gid[i]].retired) return;
-    const int flags = gd[gid[i]].flags;
+    const int g = gid[i];
+    const int flags = gd[g].flags;
     ForceSum<R> sum;
-    // one incidence in flight per lane: unrolled, the wave items measured
-    // slower (more registers, fewer waves resident)
-    attract_terms<R, 1>(it.y + lane, it.z, 64, flags & GF_LINLOG, flags & GF_ADJUST, pos[i],
+    attract_terms<R, U>(it.y + lane, it.z, 64, flags & GF_LINLOG, flags & GF_ADJUST, pos[i],
                         size[i], pos, size, inc_node, inc_coef, sum);
     group_sum<64>(sum.x, sum.y);
-    if (lane != 0) return;
+    if (lane != 0 || gs[g].retired) return;
     if (it.w < 0) {
         double2 f = F[i];
         f.x += sum.x;
@@ -1321,7 +1326,7 @@ template <typename R> struct EngineImpl final : Engine {
     long long I = 0;
     std::vector<GraphDesc> h_gd;
     std::vector<Tile> h_tiles;
-    std::vector<int> h_class[2];
+    std::vector<int4> h_class[2];
     std::vector<int4> h_items, h_hubs;
     int nslots = 0;
     bool any_adjust = false;
@@ -1339,7 +1344,7 @@ template <typename R> struct EngineImpl final : Engine {
     DevBuf<R> d_mass, d_size, d_coef;
     DevBuf<uint8_t> d_fixed;
     DevBuf<int> d_inc_off, d_inc_node;
-    DevBuf<int> d_class[2];
+    DevBuf<int4> d_class[2];
     DevBuf<int4> d_items, d_hubs;
     DevBuf<double2> d_hubpart;
     DevBuf<V4> d_tbox, d_gbox, d_psort, d_nd;
@@ -1363,6 +1368,9 @@ template <typename R> struct EngineImpl final : Engine {
     // two iterations (one period of the force buffers) captured once
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+
+    // launch shapes (none changes a result)
+    fa2_tuning shape = {{4, 4, 4}, 256};
 
     ~EngineImpl() override
     {
@@ -1391,18 +1399,32 @@ template <typename R> struct EngineImpl final : Engine {
     int set_state(int graph, const double *pos, const double *fold, double speed,
                   double eff) override;
 
+    int tune(int force, fa2_tuning *chosen, double *times) override;
+    int get_tuning(fa2_tuning *t) override;
+    int set_tuning(const fa2_tuning *t) override;
+
     void enqueue_iteration(std::vector<hipEvent_t> *ev);
+    void mark(std::vector<hipEvent_t> *ev);
+    // phases 1-4: codes, sort, tree of the current positions (scratch
+    // buffers only)
+    void enqueue_tree(std::vector<hipEvent_t> *ev);
+    // the attraction kernel of class c at unroll u, adding into Fc
+    void launch_attract_class(int c, int u, double2 *Fc);
+    void drop_graph();
 
     // the repulsion kernel for this batch: with sizes or not, counting or
-    // not, record offsets in 32 or 64 bits
-    template <bool COUNT> void launch_repulse(double2 *Fout, unsigned long long *cnt)
+    // not, record offsets in 32 or 64 bits; `threads` per workgroup (64,
+    // 128 or 256; each wave walks for the same 64 nodes whatever the size)
+    template <bool COUNT> void launch_repulse(double2 *Fout, unsigned long long *cnt,
+                                              int threads = 0)
     {
         const bool wide = ((size_t) NI + (size_t) N) * sizeof(TNode<R>) >= ((size_t) 1 << 32);
-        const unsigned blocks = ceil_div(N, 256);
-#define FA2_REPULSE(ADJ, WIDE)                                                                  \
-    k_repulse<R, ADJ, COUNT, WIDE><<<blocks, 256, 0, stream>>>(N, NI, d_gid.p, d_gd.p, d_gs.p,   \
-                                                               d_psort.p, d_tn.p, d_valA.p, Fout, \
-                                                               cnt)
+        if (threads <= 0) threads = shape.repulse_threads;
+        const unsigned blocks = ceil_div(N, threads);
+#define FA2_REPULSE(ADJ, WIDE)                                                                   \
+    k_repulse<R, ADJ, COUNT, WIDE><<<blocks, threads, 0, stream>>>(N, NI, d_gid.p, d_gd.p, d_gs.p, \
+                                                                   d_psort.p, d_tn.p, d_valA.p,    \
+                                                                   Fout, cnt)
         if (any_adjust) {
             if (wide) FA2_REPULSE(true, true);
             else FA2_REPULSE(true, false);
@@ -1559,9 +1581,9 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         const int e0 = inc_off[i], e1 = inc_off[i + 1], deg = e1 - e0;
         if (deg == 0) continue;
         if (deg <= 16) {
-            h_class[0].push_back(i);
+            h_class[0].push_back(make_int4(i, e0, e1, gid[i]));
         } else if (deg <= 256) {
-            h_class[1].push_back(i);
+            h_class[1].push_back(make_int4(i, e0, e1, gid[i]));
         } else if (deg <= kChunk) {
             h_items.push_back(make_int4(i, e0, e1, -1));
         } else {
@@ -1571,6 +1593,23 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
             h_hubs.push_back(make_int4(i, first, nslots - first, 0));
         }
     }
+    // lanes of one wave run as many rounds as its longest list: order each
+    // class by degree band (4 bands, longest first), node order within a band
+    auto by_band = [](std::vector<int4> &v, int width) {
+        auto slot = [width](const int4 &a) { return 3 - (a.z - a.y - 1) / width; };
+        size_t at[4] = {0, 0, 0, 0};
+        for (const int4 &a : v) at[slot(a)]++;
+        for (size_t k = 0, sum = 0; k < 4; k++) {
+            const size_t c = at[k];
+            at[k] = sum;
+            sum += c;
+        }
+        std::vector<int4> out(v.size());
+        for (const int4 &a : v) out[at[slot(a)]++] = a;
+        v.swap(out);
+    };
+    by_band(h_class[0], 4);
+    by_band(h_class[1], 64);
 
     HipStatus hs;
     auto up = [&](void *dst, const void *src, size_t bytes) {
@@ -1653,7 +1692,7 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         up(d_inc_node.p, inc_node.data(), (size_t) I * sizeof(int));
         up(d_coef.p, coef.data(), (size_t) I * sizeof(R));
     }
-    for (int c = 0; c < 2; c++) up(d_class[c].p, h_class[c].data(), h_class[c].size() * sizeof(int));
+    for (int c = 0; c < 2; c++) up(d_class[c].p, h_class[c].data(), h_class[c].size() * sizeof(int4));
     up(d_items.p, h_items.data(), h_items.size() * sizeof(int4));
     up(d_hubs.p, h_hubs.data(), h_hubs.size() * sizeof(int4));
     if (hs.ok())
@@ -1674,26 +1713,75 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     return 0;
 }
 
+template <typename R> void EngineImpl<R>::mark(std::vector<hipEvent_t> *ev)
+{
+    if (!ev) return;
+    hipEvent_t e;
+    if (ev_pool.empty()) {
+        (void) hipEventCreate(&e);
+    } else {
+        e = ev_pool.back();
+        ev_pool.pop_back();
+    }
+    (void) hipEventRecord(e, stream);
+    ev->push_back(e);
+}
+
+template <typename R> void EngineImpl<R>::launch_attract_class(int c, int u, double2 *Fc)
+{
+    const int cnt = c < 2 ? (int) h_class[c].size() : (int) h_items.size();
+    if (cnt == 0) return;
+#define FA2_ATTRACT(U)                                                                          \
+    do {                                                                                        \
+        if (c == 0)                                                                             \
+            k_attract<R, 1, U><<<ceil_div(cnt, 256), 256, 0, stream>>>(                         \
+                cnt, d_class[0].p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_node.p, d_coef.p, Fc); \
+        else if (c == 1)                                                                        \
+            k_attract<R, 16, U><<<ceil_div(cnt, 16), 256, 0, stream>>>(                         \
+                cnt, d_class[1].p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_node.p, d_coef.p, Fc); \
+        else                                                                                    \
+            k_attract_items<R, U><<<ceil_div(cnt, 4), 256, 0, stream>>>(                        \
+                cnt, d_items.p, d_gid.p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_node.p,        \
+                d_coef.p, Fc, d_hubpart.p);                                                     \
+    } while (0)
+    switch (u) {
+    case 1: FA2_ATTRACT(1); break;
+    case 2: FA2_ATTRACT(2); break;
+    case 8: FA2_ATTRACT(8); break;
+    default: FA2_ATTRACT(4); break;
+    }
+#undef FA2_ATTRACT
+}
+
 template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent_t> *ev)
 {
-    auto mark = [&]() {
-        if (!ev) return;
-        hipEvent_t e;
-        if (ev_pool.empty()) {
-            (void) hipEventCreate(&e);
-        } else {
-            e = ev_pool.back();
-            ev_pool.pop_back();
-        }
-        (void) hipEventRecord(e, stream);
-        ev->push_back(e);
-    };
     double2 *Fc = d_F[parity].p, *Fo = d_F[1 - parity].p;
-    mark();
+    enqueue_tree(ev);
+    // 5: repulsion
+    launch_repulse<false>(Fc, nullptr);
+    mark(ev);
+    // 6: attraction (gravity was applied with the repulsion)
+    for (int c = 0; c < 3; c++) launch_attract_class(c, shape.attract_unroll[c], Fc);
+    if (!h_hubs.empty())
+        k_attract_fix<R><<<ceil_div((int) h_hubs.size(), 4), 256, 0, stream>>>(
+            (int) h_hubs.size(), d_hubs.p, d_gid.p, d_gs.p, d_hubpart.p, Fc);
+    mark(ev);
+    // 7-8: swinging, speed, displacement
+    k_swing<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_mass.p, d_fixed.p, Fc, Fo, d_part.p);
+    k_speed<<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_part.p, d_gs.p);
+    k_apply<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, d_mass.p, d_fixed.p, Fc, Fo,
+                                      d_pos.p, d_size.p, d_node4.p, d_acc.p, d_tbox.p);
+    mark(ev);
+    parity = 1 - parity;
+}
+
+template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *ev)
+{
+    mark(ev);
     // 1-2: codes
     k_graph_bbox<R><<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_tbox.p, d_gbox.p);
     k_morton<R><<<T, 256, 0, stream>>>(d_tiles.p, d_pos.p, d_gbox.p, d_keyA.p, d_valA.p);
-    mark();
+    mark(ev);
     // 3: sort, A -> B -> A -> B -> A; the last pass gathers psort
     for (int pass = 0; pass < 4; pass++) {
         uint32_t *ki = pass % 2 ? d_keyB.p : d_keyA.p, *vi = pass % 2 ? d_valB.p : d_valA.p;
@@ -1713,7 +1801,7 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
                 d_tiles.p, d_gd.p, ki, vi, 8 * pass, d_hist.p, d_rowtot.p, ko, vo, d_node4.p,
                 d_psort.p);
     }
-    mark();
+    mark(ev);
     // 4: tree
     if (NI > 0) {
         k_karras<R><<<ceil_div(NI, 256), 256, 0, stream>>>(
@@ -1734,38 +1822,7 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
                                                             d_nd.p, d_tn.p);
         }
     }
-    mark();
-    // 5: repulsion
-    launch_repulse<false>(Fc, nullptr);
-    mark();
-    // 6: attraction (gravity was applied with the repulsion)
-    {
-        const int c0 = (int) h_class[0].size(), c1 = (int) h_class[1].size();
-        const int ci = (int) h_items.size(), ch = (int) h_hubs.size();
-        if (c0)
-            k_attract<R, 1><<<ceil_div(c0, 256), 256, 0, stream>>>(
-                c0, d_class[0].p, d_gid.p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_off.p, d_inc_node.p,
-                d_coef.p, Fc);
-        if (c1)
-            k_attract<R, 16><<<ceil_div(c1, 16), 256, 0, stream>>>(
-                c1, d_class[1].p, d_gid.p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_off.p, d_inc_node.p,
-                d_coef.p, Fc);
-        if (ci)
-            k_attract_items<R><<<ceil_div(ci, 4), 256, 0, stream>>>(
-                ci, d_items.p, d_gid.p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_node.p, d_coef.p, Fc,
-                d_hubpart.p);
-        if (ch)
-            k_attract_fix<R><<<ceil_div(ch, 4), 256, 0, stream>>>(ch, d_hubs.p, d_gid.p, d_gs.p,
-                                                                  d_hubpart.p, Fc);
-    }
-    mark();
-    // 7-8: swinging, speed, displacement
-    k_swing<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_mass.p, d_fixed.p, Fc, Fo, d_part.p);
-    k_speed<<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_part.p, d_gs.p);
-    k_apply<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, d_mass.p, d_fixed.p, Fc, Fo,
-                                      d_pos.p, d_size.p, d_node4.p, d_acc.p, d_tbox.p);
-    mark();
-    parity = 1 - parity;
+    mark(ev);
 }
 
 // Record two iterations into a HIP graph (no memory set or copy is part of
@@ -2150,6 +2207,147 @@ template <typename R> int EngineImpl<R>::count_work(unsigned long long out[4])
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Autotuner. Candidates differ only in launch shape, never in what a lane
+// adds or in which order, so any choice gives bit-identical layouts. The
+// choice is cached per process by device, precision, sizes (adjust or not)
+// and the binary magnitude of the node count and of each attraction class.
+// ---------------------------------------------------------------------------
+
+namespace {
+struct TuneKey {
+    int device, real_bytes, adjust, n, c0, c1, c2;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(device, real_bytes, adjust, n, c0, c1, c2) <
+               std::tie(o.device, o.real_bytes, o.adjust, o.n, o.c0, o.c1, o.c2);
+    }
+};
+std::mutex tune_mutex;
+std::map<TuneKey, fa2_tuning> tune_cache;
+
+int magnitude(long long v)
+{
+    int b = 0;
+    while (v > 0) {
+        b++;
+        v >>= 1;
+    }
+    return b;
+}
+
+bool valid_unroll(int u) { return u == 1 || u == 2 || u == 4 || u == 8; }
+} // namespace
+
+template <typename R> void EngineImpl<R>::drop_graph()
+{
+    if (graph_exec) (void) hipGraphExecDestroy(graph_exec);
+    if (graph) (void) hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+}
+
+template <typename R> int EngineImpl<R>::get_tuning(fa2_tuning *t)
+{
+    *t = shape;
+    return 0;
+}
+
+template <typename R> int EngineImpl<R>::set_tuning(const fa2_tuning *t)
+{
+    for (int c = 0; c < 3; c++)
+        if (!valid_unroll(t->attract_unroll[c])) {
+            set_message("attraction unroll %d: 1, 2, 4 or 8", t->attract_unroll[c]);
+            return 1;
+        }
+    if (t->repulse_threads != 64 && t->repulse_threads != 128 && t->repulse_threads != 256) {
+        set_message("traversal workgroup of %d threads: 64, 128 or 256", t->repulse_threads);
+        return 1;
+    }
+    (void) hipSetDevice(device);
+    (void) hipStreamSynchronize(stream); // a captured graph may still be running
+    shape = *t;
+    drop_graph(); // recaptured with the new shapes at the next run
+    return 0;
+}
+
+template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, double *times)
+{
+    (void) hipSetDevice(device);
+    const TuneKey key{device, (int) sizeof(R), any_adjust ? 1 : 0, magnitude(N),
+                      magnitude((long long) h_class[0].size()),
+                      magnitude((long long) h_class[1].size()), magnitude((long long) h_items.size())};
+    if (!force) {
+        std::lock_guard<std::mutex> lock(tune_mutex);
+        auto it = tune_cache.find(key);
+        if (it != tune_cache.end()) {
+            if (set_tuning(&it->second)) return 1;
+            if (chosen) *chosen = shape;
+            return 0;
+        }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    // the tree of the current positions; the candidates write only the force
+    // buffer that the next iteration overwrites first
+    double2 *Fs = d_F[parity].p;
+    if (err == hipSuccess) {
+        enqueue_tree(nullptr);
+        err = hipGetLastError();
+    }
+    auto best_of_3 = [&](auto &&launch) -> double {
+        double best = 0.0;
+        for (int k = 0; k < 3 && err == hipSuccess; k++) {
+            err = hipEventRecord(e0, stream);
+            if (err != hipSuccess) break;
+            launch();
+            err = hipGetLastError();
+            if (err == hipSuccess) err = hipEventRecord(e1, stream);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+            if (err == hipSuccess && (k == 0 || ms < best)) best = ms;
+        }
+        return best;
+    };
+    double t[FA2_TUNE_NTIMES] = {0};
+    fa2_tuning pick = shape;
+    static const int threads[3] = {64, 128, 256}, unroll[4] = {1, 2, 4, 8};
+    int bk = 0;
+    for (int k = 0; k < 3; k++) {
+        t[k] = best_of_3([&] { launch_repulse<false>(Fs, nullptr, threads[k]); });
+        if (t[k] < t[bk]) bk = k;
+    }
+    pick.repulse_threads = threads[bk];
+    for (int c = 0; c < 3; c++) {
+        const size_t cnt = c < 2 ? h_class[c].size() : h_items.size();
+        if (cnt == 0) continue;
+        int bu = -1;
+        for (int k = 0; k < 4; k++) {
+            double &tk = t[3 + 4 * c + k];
+            tk = best_of_3([&] { launch_attract_class(c, unroll[k], Fs); });
+            if (bu < 0 || tk < t[3 + 4 * c + bu]) bu = k;
+        }
+        pick.attract_unroll[c] = unroll[bu];
+    }
+    if (e0) (void) hipEventDestroy(e0);
+    if (e1) (void) hipEventDestroy(e1);
+    if (err != hipSuccess) {
+        set_hip_error("could not time the launch shapes", err);
+        return 1;
+    }
+    if (set_tuning(&pick)) return 1;
+    {
+        std::lock_guard<std::mutex> lock(tune_mutex);
+        tune_cache[key] = pick;
+    }
+    if (chosen) *chosen = pick;
+    if (times)
+        for (int k = 0; k < FA2_TUNE_NTIMES; k++) times[k] = t[k];
+    return 0;
+}
+
 template <typename R>
 int EngineImpl<R>::set_state(int graph, const double *pos, const double *fold, double speed,
                              double eff)
@@ -2210,7 +2408,7 @@ template <typename R> static size_t footprint_of(long long n, long long incidenc
     b += (size_t) n * (sizeof(int) /*gid*/ + sizeof(V2) /*pos*/ + sizeof(V4) /*node4*/ +
                        2 * sizeof(double2) /*F*/ + (Acc<R>::separate ? sizeof(double2) : 0) +
                        3 * sizeof(R) /*mass, size*/ + 1 /*fixed*/ + sizeof(int) /*inc_off*/ +
-                       sizeof(int) /*class lists*/ + sizeof(V4) /*psort*/ +
+                       sizeof(int4) /*class lists*/ + sizeof(V4) /*psort*/ +
                        4 * sizeof(uint32_t) /*keys, values*/ + sizeof(int) /*parent_leaf*/ +
                        sizeof(int) /*ready*/);
     b += (size_t) ni * (sizeof(int) /*gi*/ + 2 * sizeof(V4) /*nd*/ + sizeof(int2) /*child*/ +

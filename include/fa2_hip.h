@@ -144,6 +144,37 @@ int fa2_batch_count_work(fa2_batch *b, unsigned long long out[4]);
 int fa2_batch_set_state(fa2_batch *b, int g, const double *pos, const double *f_old,
                         double speed, double speed_efficiency);
 
+/* Launch shapes of a batch's kernels. None of them changes a result, only
+ * the speed: attract_unroll[c] is the number of incidences whose loads each
+ * lane has in flight at once (1, 2, 4 or 8) in the attraction kernel of
+ * class c (c = 0: nodes of 1..16 incidences, one lane each; 1: 17..256,
+ * 16 lanes each; 2: longer lists, one wave per chunk), and
+ * repulse_threads the workgroup size of the Barnes-Hut traversal (64, 128
+ * or 256). The reference has no counterpart (its work sizes are fixed in
+ * src/gpu_full.c:193-198). */
+typedef struct fa2_tuning {
+    int attract_unroll[3];
+    int repulse_threads;
+} fa2_tuning;
+
+/* Timings behind fa2_batch_tune: the traversal at 64, 128 and 256 threads,
+ * then for each attraction class the unrolls 1, 2, 4 and 8 (ms; 0 where a
+ * class is empty). */
+#define FA2_TUNE_NTIMES 15
+
+/* Autotuner: builds the tree of the batch's current positions and times
+ * every candidate launch shape on it, with HIP events on the batch's stream
+ * (best of three launches each), then keeps the fastest for the following
+ * iterations. The layout state is left as it was: the candidates write only
+ * into the force buffer that the next iteration overwrites. Choices are
+ * cached per process by device, precision and the batch's size class; with
+ * force == 0 a cached choice is reused without timing (times then stay
+ * untouched). chosen and times (FA2_TUNE_NTIMES) may be NULL. 0 on success. */
+int fa2_batch_tune(fa2_batch *b, int force, fa2_tuning *chosen, double *times);
+int fa2_batch_get_tuning(fa2_batch *b, fa2_tuning *t);
+/* 1 with a message on a value out of the ranges above. */
+int fa2_batch_set_tuning(fa2_batch *b, const fa2_tuning *t);
+
 /* The HIP stream of the batch (hipStream_t), for callers that time kernels
  * with events on it. */
 void *fa2_batch_stream(fa2_batch *b);

@@ -1,0 +1,108 @@
+"""The autotuner of a batch's launch shapes (fa2_batch_tune,
+fa2_batch_set_tuning of include/fa2_hip.h).
+
+Gates: every candidate shape gives positions bit-identical to the default
+shapes, with the iterations replayed from the captured HIP graph (which is
+recaptured after each change) and launched kernel by kernel; tuning leaves
+the layout state untouched; the tuner keeps the fastest candidate it timed;
+a server batch above the autotune size gives each graph's layout alone.
+"""
+import numpy as np
+import pytest
+
+from forceatlas2_rc_amd import Batch, GPUError, Server, graphs
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [((1, 1, 1), 64), ((2, 2, 2), 128), ((8, 8, 8), 256), ((4, 1, 8), 64),
+          ((8, 4, 2), 128)]
+
+
+def _graphs():
+    """Every attraction class occupied: R-MAT hubs of thousands of
+    incidences (wave items and hub partials), preferential attachment
+    (16-lane class), a sparse block model (one lane per node); linlog,
+    dissuade_hubs and node sizes mixed in."""
+    n, e = graphs.rmat(14, 120_000, seed=5)
+    r = 10 * np.sqrt(n)
+    a = dict(n=n, edges=e, init=graphs.uniform_positions(n, -r, r, 6), linlog=True,
+             dissuade_hubs=True)
+    n2, e2 = graphs.barabasi_albert(6000, 12, seed=7)
+    b = dict(n=n2, edges=e2, init=graphs.uniform_positions(n2, -800, 800, 8))
+    n3, e3 = graphs.sbm(3000, 6, deg_in=4.0, deg_out=1.0, seed=9)
+    c = dict(n=n3, edges=e3, init=graphs.uniform_positions(n3, -500, 500, 10), adjust=True,
+             size=np.random.default_rng(11).uniform(1.0, 3.0, n3))
+    return [a, b, c]
+
+
+def _layout(gs, iters, shape=None, profile=False, tune=False, precision="single"):
+    with Batch(gs, profile=profile, autotune=False, precision=precision) as b:
+        if shape is not None:
+            b.set_tuning(*shape)
+        if tune:
+            b.tune(force=True)
+        b.run(iters)
+        b.sync()
+        return [b.positions(g) for g in range(len(gs))]
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_every_shape_gives_the_same_layout(precision):
+    gs = _graphs()
+    ref = _layout(gs, 7, precision=precision)
+    for shape in SHAPES:
+        for profile in (False, True):
+            out = _layout(gs, 7, shape, profile, precision=precision)
+            for g in range(len(gs)):
+                assert np.array_equal(out[g], ref[g]), (shape, profile, g)
+
+
+def test_tuning_keeps_the_state_and_the_fastest():
+    gs = _graphs()
+    ref = _layout(gs, 6)
+    with Batch(gs, autotune=False) as b:
+        b.run(3)
+        chosen, times = b.tune(force=True)
+        assert times is not None and np.all(times[:3] > 0)
+        assert np.all(times[3:] > 0)  # every class is occupied
+        threads = (64, 128, 256)
+        assert chosen["repulse_threads"] == threads[int(np.argmin(times[:3]))]
+        for c in range(3):
+            u = (1, 2, 4, 8)[int(np.argmin(times[3 + 4 * c:7 + 4 * c]))]
+            assert chosen["attract_unroll"][c] == u
+        assert b.tuning == chosen
+        b.run(3)
+        b.sync()
+        for g in range(len(gs)):
+            assert np.array_equal(b.positions(g), ref[g])
+        # a second batch of the same size class reuses the choice untimed
+        with Batch(gs, autotune=False) as b2:
+            again, t2 = b2.tune(force=False)
+            assert t2 is None and again == chosen
+    assert np.array_equal(_layout(gs, 6, tune=True)[0], ref[0])
+
+
+def test_tuning_errors():
+    gs = _graphs()[1:2]
+    with Batch(gs, autotune=False) as b:
+        for bad in (((3, 4, 4), 256), ((4, 4, 4), 96), ((4, 0, 4), 64)):
+            with pytest.raises(GPUError):
+                b.set_tuning(*bad)
+        assert b.tuning == dict(attract_unroll=(4, 4, 4), repulse_threads=256)
+
+
+def test_server_batches_above_the_autotune_size():
+    reqs = []
+    for k, n in enumerate((70_000, 5_000)):  # the first alone is above the size
+        n, e = graphs.barabasi_albert(n, 4, seed=60 + k)
+        r = 10 * np.sqrt(n)
+        reqs.append(dict(n=n, edges=e, init=graphs.uniform_positions(n, -r, r, 70 + k),
+                         iterations=9 + 4 * k, linlog=k == 1))
+    with Server(max_batches=1, chunk=5) as srv:
+        ids = [srv.submit(g) for g in reqs]
+        out = [srv.result(j)[0] for j in ids]
+    for g, pos in zip(reqs, out):
+        with Batch([g], autotune=False) as b:
+            b.run(g["iterations"])
+            b.sync()
+            assert np.array_equal(pos, b.positions(0))
