@@ -50,6 +50,10 @@ namespace fa2 {
 constexpr int kNoRecord = -1;
 __device__ inline int record_of(int r, int NI) { return r >= 0 ? r : NI + ~r; }
 constexpr int kLeafSize = 8; // regions of at most 8 nodes are visited node by node
+// the lengths of the summary unit lists (see k_karras) are counted with
+// atomics from the whole device: one counter per 128-byte line, so that they
+// spread over the L2 channels
+constexpr int kCountStride = 32;
 
 // ---------------------------------------------------------------------------
 // Device descriptors
@@ -175,8 +179,11 @@ template <typename R>
 __global__ void __launch_bounds__(256)
 k_morton(const Tile *__restrict__ tiles, const typename Vec<R>::v2 *__restrict__ pos,
          const typename Vec<R>::v4 *__restrict__ gbox, uint32_t *__restrict__ key,
-         uint32_t *__restrict__ val)
+         uint32_t *__restrict__ val, int UT, int *__restrict__ ucnt)
 {
+    // the lengths of the summary unit lists, filled by k_karras
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < UT; q += gridDim.x * blockDim.x)
+        ucnt[q * kCountStride] = 0;
     const Tile t = tiles[blockIdx.x];
     const typename Vec<R>::v4 bb = gbox[t.g];
     const R ex = bb.z - bb.x, ey = bb.w - bb.y;
@@ -419,16 +426,46 @@ __device__ inline int kdelta(const uint32_t *__restrict__ ks, int n, int a, int 
     return __clz((int) (x ^ y));
 }
 
+// Summaries are merged bottom-up in levels of units of sorted positions,
+// aligned to the graph's first position. Level 1: blocks of agg_block<R>()
+// positions (half a tile in single precision, a quarter in double), one
+// workgroup each, in LDS. Level L >= 2: units of agg_block<R>() * 16^(L-1)
+// positions. A region belongs to the smallest unit that holds its whole
+// range; the regions of one unit of level >= 2 are listed by k_karras and
+// merged by one workgroup in LDS (k_aggregate_unit), level after level.
+//
+// Capacity: a region of a unit of level L crosses one of the 15 boundaries
+// between the unit's 16 sub-units. The regions whose ranges contain one
+// boundary form a nested chain in which the common prefix of the (key,
+// position) pairs strictly shrinks, so the chain has at most 64 regions, and
+// a unit at most 15 * 64 = 960: the lists of kUnitCap entries never overflow.
+constexpr int kUnitFan = 16;
+constexpr int kUnitCap = 1024;
+
+constexpr int kMaxLevels = 8; // units of 512 * 16^6 positions exceed any graph
+
+template <typename R> constexpr int agg_block() { return sizeof(R) == 4 ? kTile / 2 : kTile / 4; }
+template <typename R> constexpr int agg_shift() { return sizeof(R) == 4 ? 10 : 9; }
+static_assert(agg_block<float>() == 1 << agg_shift<float>() &&
+                  agg_block<double>() == 1 << agg_shift<double>() && kUnitFan == 16,
+              "summary units are powers of two, 16 sub-units each");
+
+struct Unit {
+    int g, index, pad0, pad1; // graph, index of its list
+};
+
 // Region t of graph gi[t] (Karras 2012, as pipeline.cl:226-270), with global
 // node references. Also records, for the gap after each sorted position, the
 // node a traversal visits next (the right child of the region split there),
-// and clears the arrival counters.
+// and lists the region in its summary unit when that is of level >= 2:
+// xslot[t] = level << 16 | position in the unit's list, 0 inside a block.
 template <typename R>
 __global__ void k_karras(int NI, const int *__restrict__ gi, const GraphDesc *__restrict__ gd,
                          const uint32_t *__restrict__ ks, int2 *__restrict__ child,
                          TNode<R> *__restrict__ tn, int *__restrict__ parent_leaf,
                          int *__restrict__ parent_int, int *__restrict__ escape,
-                         int *__restrict__ visits, int NS, int *__restrict__ rcnt)
+                         const int *__restrict__ ubase, int *__restrict__ ucnt,
+                         int *__restrict__ lists, int *__restrict__ xslot)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= NI) return;
@@ -463,8 +500,26 @@ __global__ void k_karras(int NI, const int *__restrict__ gi, const GraphDesc *__
     if (i == 0) parent_int[t] = -1;
     escape[S - g] = right;
     tn[t].l = make_int4(record_of(left, NI), 0, base + first, base + last);
-    visits[t] = 0;
-    for (int q = t; q <= NS; q += NI) rcnt[q] = 0;
+    // units are powers of two: level L splits positions at bit
+    // log2(block) + 4 (L - 1); positions stay below 2^26, so the shift stays
+    // below 32
+    int xs = 0;
+    int sh = agg_shift<R>();
+    if ((first >> sh) != (last >> sh)) {
+        int L = 1;
+        do {
+            L++;
+            sh += 4;
+        } while ((first >> sh) != (last >> sh) && L < kMaxLevels - 1);
+        const int ub = ubase[g * kMaxLevels + L];
+        if (ub >= 0) { // always: the graph is longer than a unit of level L - 1
+            const int unit = ub + (first >> sh);
+            const int q = atomicAdd(&ucnt[unit * kCountStride], 1);
+            if (q < kUnitCap) lists[(size_t) unit * kUnitCap + q] = t;
+            xs = L << 16 | min(q, 0xFFFF);
+        }
+    }
+    xslot[t] = xs;
 }
 
 template <typename R> struct Summ {
@@ -504,80 +559,10 @@ template <typename R> __device__ inline typename Vec<R>::v4 region_record(const 
     return mk4<R>(cx, cy, r.m, (R) 2 * (w * w + h * h));
 }
 
-// One level of the climbs of the second and third levels: the summaries of
-// region t from those of its two children, merged left then right
-// (pipeline.cl:276-305), its traversal record (pipeline.cl:312-323) with its
-// skip link, and its parent returned with the parent's range.
-// Every load that depends only on t is issued first, and the loads that
-// depend on those together, before any store: a level costs two round trips
-// to memory instead of one per load.
-template <typename R>
-__device__ inline int merge_and_parent(int t, int g, int gend, int NI,
-                                       const typename Vec<R>::v4 *__restrict__ psort,
-                                       const int2 *__restrict__ child,
-                                       const int *__restrict__ parent_int,
-                                       const int *__restrict__ escape,
-                                       typename Vec<R>::v4 *__restrict__ nd,
-                                       TNode<R> *__restrict__ tn, int4 &prange)
-{
-    using V4 = typename Vec<R>::v4;
-    const int2 c = child[t];
-    const int last = tn[t].l.w;
-    const int p = parent_int[t];
-    V4 au, av, bu, bv;
-    if (c.x < 0) au = psort[~c.x];
-    else {
-        au = nd[2 * c.x];
-        av = nd[2 * c.x + 1];
-    }
-    if (c.y < 0) bu = psort[~c.y];
-    else {
-        bu = nd[2 * c.y];
-        bv = nd[2 * c.y + 1];
-    }
-    const int esc = last == gend - 1 ? 0 : escape[last - g];
-    prange = p >= 0 ? tn[p].l : make_int4(0, 0, 0, 0);
-    const Summ<R> a = c.x < 0 ? leaf_summ<R>(au) : Summ<R>{au.x, au.y, au.z, au.w, av.x, av.y, av.z};
-    const Summ<R> b = c.y < 0 ? leaf_summ<R>(bu) : Summ<R>{bu.x, bu.y, bu.z, bu.w, bv.x, bv.y, bv.z};
-    const Summ<R> r = merge_summ<R>(a, b);
-    nd[2 * t] = mk4<R>(r.x0, r.y0, r.x1, r.y1);
-    nd[2 * t + 1] = mk4<R>(r.m, r.mx, r.my, (R) 0);
-    tn[t].c = region_record<R>(r);
-    tn[t].l.y = last == gend - 1 ? kNoRecord : record_of(esc, NI);
-    return p;
-}
-
-// Leaf to root in three levels, the second child to arrive at a region
-// merging it:
-//  1. each tile (2048 sorted nodes) by one workgroup, arrival counters in
-//     LDS with workgroup-scope acquire/release;
-//  2. the regions spanning several tiles of one super-tile (64 tiles), by
-//     one workgroup per super-tile;
-//  3. the few regions spanning several super-tiles, by one workgroup.
-// A region spanning a wider block than the current level only counts the
-// arrival in HBM; when both of its children are complete it joins the ready
-// list of the next level. No agent-scope fence is needed: each kernel
-// boundary publishes the summaries to the next level. The super-tile of
-// sorted position p of graph g is sbase_g + (p - noff_g) / kSuper.
-
-constexpr int kSuper = 64 * kTile;
-
-struct Super {
-    int g, lo, hi, pad; // graph, sorted positions [lo, hi)
-};
-
-__device__ inline void push_ready(const GraphDesc &D, int t, int first, int last, int NS, int N,
-                                  int *ready, int *rcnt)
-{
-    const int s0 = D.pad + (first - D.noff) / kSuper, s1 = D.pad + (last - D.noff) / kSuper;
-    if (s0 == s1) {
-        const int q = atomicAdd(&rcnt[s0], 1);
-        ready[D.noff + (s0 - D.pad) * kSuper + q] = t;
-    } else {
-        const int q = atomicAdd(&rcnt[NS], 1);
-        ready[N + q] = t;
-    }
-}
+// Leaf to root, the second child to arrive at a region merging it. The
+// reference climbs with atomic_inc and no fence (pipeline.cl:294-302); here
+// every level runs in the LDS of one workgroup, and each kernel boundary
+// publishes a level's summaries to the next: no agent-scope fence is needed.
 
 // Level 1 in LDS. The workgroup of a block of sorted positions [lo, hi)
 // (half a tile in single precision, a quarter in double: 60 and 52 KB of LDS,
@@ -591,15 +576,12 @@ __device__ inline void push_ready(const GraphDesc &D, int t, int first, int last
 // whose parent leaves the block (read by the next levels).
 constexpr int kOutside = 1 << 24;
 
-template <typename R> constexpr int agg_block() { return sizeof(R) == 4 ? kTile / 2 : kTile / 4; }
-
 template <typename R>
 __global__ void __launch_bounds__(1024)
 k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
                  const typename Vec<R>::v4 *__restrict__ psort, const int2 *__restrict__ child,
                  const int *__restrict__ parent_leaf, const int *__restrict__ parent_int,
-                 const int *__restrict__ escape, int NS, int N, int NI, int *gvis, int *ready,
-                 int *rcnt, typename Vec<R>::v4 *nd, TNode<R> *tn)
+                 const int *__restrict__ escape, int NI, typename Vec<R>::v4 *nd, TNode<R> *tn)
 {
     using V4 = typename Vec<R>::v4;
     constexpr int B = agg_block<R>();
@@ -656,11 +638,7 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
                                 ? __hip_atomic_fetch_add(&s_vis[q], 1, __ATOMIC_ACQ_REL,
                                                          __HIP_MEMORY_SCOPE_WORKGROUP)
                                 : kOutside;
-            if (old >= kOutside) { // the region leaves the block: the next levels
-                const int4 l = tn[t].l;
-                if (atomicAdd(&gvis[t], 1) == 1) push_ready(D, t, l.z, l.w, NS, N, ready, rcnt);
-                break;
-            }
+            if (old >= kOutside) break; // the region leaves the block: the next levels
             if (old == 0) break; // the other child is not summarised yet
             const int2 c = s_child[q];
             Summ<R> a, b;
@@ -709,60 +687,116 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     }
 }
 
+// One unit of level `level` >= 2: its listed regions are staged in LDS, one
+// per thread, with their children's places (in the list, or summaries of
+// blocks, lower units and leaves loaded from HBM up front); a region with
+// both children from outside is merged at once and starts a climb, and the
+// climb then runs through LDS arrival counters as in level 1. Written to
+// HBM: every listed region's traversal record, and the summaries of those
+// whose parent lies in a higher unit.
 template <typename R>
 __global__ void __launch_bounds__(1024)
-k_aggregate_super(const Super *__restrict__ supers, const GraphDesc *__restrict__ gd,
-                  const typename Vec<R>::v4 *__restrict__ psort, const int2 *__restrict__ child,
-                  const int *__restrict__ parent_int, const int *__restrict__ escape, int NS,
-                  int N, int NI, int *gvis, int *ready, int *rcnt, typename Vec<R>::v4 *nd,
-                  TNode<R> *tn)
+k_aggregate_unit(const Unit *__restrict__ units, int level, const GraphDesc *__restrict__ gd,
+                 GraphState *__restrict__ gs, const typename Vec<R>::v4 *__restrict__ psort,
+                 const int2 *__restrict__ child, const int *__restrict__ parent_int,
+                 const int *__restrict__ escape, const int *__restrict__ xslot,
+                 const int *__restrict__ lists, const int *__restrict__ ucnt, int NI,
+                 typename Vec<R>::v4 *nd, TNode<R> *tn)
 {
-    const Super S = supers[blockIdx.x];
-    const int g = S.g;
-    const GraphDesc D = gd[g];
+    using V4 = typename Vec<R>::v4;
+    __shared__ int s_t[kUnitCap];
+    __shared__ int2 s_cs[kUnitCap]; // children's places in the list, -1 outside
+    __shared__ int s_ps[kUnitCap];  // parent's place, -1 in a higher unit, -2 none
+    __shared__ int s_vis[kUnitCap];
+    __shared__ int s_skip[kUnitCap];
+    __shared__ R s_sum[7][kUnitCap]; // x0, y0, x1, y1, m, mx, my
+    const Unit U = units[blockIdx.x];
+    const GraphDesc D = gd[U.g];
     const int gend = D.noff + D.n;
-    const int cnt = rcnt[blockIdx.x];
-    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-        int t = ready[S.lo + q];
-        for (int depth = 0; depth < 128; depth++) {
-            int4 l;
-            const int p = merge_and_parent<R>(t, g, gend, NI, psort, child, parent_int, escape,
-                                              nd, tn, l);
-            if (p < 0) break;
-            if (l.z < S.lo || l.w >= S.hi) { // spans several super-tiles
-                if (atomicAdd(&gvis[p], 1) == 1) push_ready(D, p, l.z, l.w, NS, N, ready, rcnt);
-                break;
-            }
-            const int old = __hip_atomic_fetch_add(&gvis[p], 1, __ATOMIC_ACQ_REL,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0) break;
-            t = p;
+    const int cnt = ucnt[U.index * kCountStride];
+    if (cnt > kUnitCap) { // cannot happen (see kUnitCap): stop the graph, never run past LDS
+        if (threadIdx.x == 0) gs[U.g].status = 2;
+        return;
+    }
+    auto put = [&](int q, const Summ<R> &r) {
+        s_sum[0][q] = r.x0;
+        s_sum[1][q] = r.y0;
+        s_sum[2][q] = r.x1;
+        s_sum[3][q] = r.y1;
+        s_sum[4][q] = r.m;
+        s_sum[5][q] = r.mx;
+        s_sum[6][q] = r.my;
+    };
+    auto get = [&](int q) {
+        return Summ<R>{s_sum[0][q], s_sum[1][q], s_sum[2][q], s_sum[3][q],
+                       s_sum[4][q], s_sum[5][q], s_sum[6][q]};
+    };
+    const int q = threadIdx.x;
+    bool start = false;
+    if (q < cnt) {
+        const int t = lists[(size_t) U.index * kUnitCap + q];
+        const int2 c = child[t];
+        const int p = parent_int[t];
+        const int last = tn[t].l.w;
+        const int xl = c.x >= 0 ? xslot[c.x] : 0, xr = c.y >= 0 ? xslot[c.y] : 0;
+        const int xp = p >= 0 ? xslot[p] : 0;
+        const int esc = last == gend - 1 ? 0 : escape[last - U.g];
+        // the summaries a child from outside the list brings (loaded for
+        // both children; unused when the child is listed)
+        V4 au, av, bu, bv;
+        if (c.x < 0) au = psort[~c.x];
+        else {
+            au = nd[2 * c.x];
+            av = nd[2 * c.x + 1];
+        }
+        if (c.y < 0) bu = psort[~c.y];
+        else {
+            bu = nd[2 * c.y];
+            bv = nd[2 * c.y + 1];
+        }
+        const bool inl = c.x >= 0 && (xl >> 16) == level, inr = c.y >= 0 && (xr >> 16) == level;
+        const Summ<R> a = c.x < 0 ? leaf_summ<R>(au) : Summ<R>{au.x, au.y, au.z, au.w, av.x, av.y, av.z};
+        const Summ<R> b = c.y < 0 ? leaf_summ<R>(bu) : Summ<R>{bu.x, bu.y, bu.z, bu.w, bv.x, bv.y, bv.z};
+        s_t[q] = t;
+        s_cs[q] = make_int2(inl ? (xl & 0xFFFF) : -1, inr ? (xr & 0xFFFF) : -1);
+        s_ps[q] = p < 0 ? -2 : ((xp >> 16) == level ? (xp & 0xFFFF) : -1);
+        s_skip[q] = last == gend - 1 ? kNoRecord : record_of(esc, NI);
+        // a child from outside is complete: counted as arrived; with one, its
+        // summary waits in the region's own slot until the other arrives
+        s_vis[q] = (inl ? 0 : 1) + (inr ? 0 : 1);
+        if (!inl && !inr) {
+            put(q, merge_summ<R>(a, b));
+            start = true;
+        } else if (!inl) {
+            put(q, a);
+        } else if (!inr) {
+            put(q, b);
         }
     }
-}
-
-template <typename R>
-__global__ void __launch_bounds__(1024)
-k_aggregate_top(const GraphDesc *__restrict__ gd, const typename Vec<R>::v4 *__restrict__ psort,
-                const int2 *__restrict__ child, const int *__restrict__ parent_int,
-                const int *__restrict__ escape, const int *__restrict__ gi, int NS, int N,
-                int NI, const int *__restrict__ ready, const int *__restrict__ rcnt, int *gvis,
-                typename Vec<R>::v4 *nd, TNode<R> *tn)
-{
-    const int cnt = rcnt[NS];
-    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-        int t = ready[N + q];
-        const int g = gi[t];
-        const int gend = gd[g].noff + gd[g].n;
-        for (int depth = 0; depth < 128; depth++) {
-            int4 l;
-            const int p = merge_and_parent<R>(t, g, gend, NI, psort, child, parent_int, escape,
-                                              nd, tn, l);
-            if (p < 0) break;
-            const int old = __hip_atomic_fetch_add(&gvis[p], 1, __ATOMIC_ACQ_REL,
+    __syncthreads();
+    if (start) {
+        int r = q;
+        for (int depth = 0; depth < 128; depth++) { // a chain has at most 64 regions
+            const int pr = s_ps[r];
+            if (pr < 0) break;
+            const int old = __hip_atomic_fetch_add(&s_vis[pr], 1, __ATOMIC_ACQ_REL,
                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0) break;
-            t = p;
+            if (old != 1) break; // the other child is not summarised yet
+            const int2 cs = s_cs[pr];
+            const Summ<R> a = get(cs.x >= 0 ? cs.x : pr), b = get(cs.y >= 0 ? cs.y : pr);
+            put(pr, merge_summ<R>(a, b));
+            r = pr;
+        }
+    }
+    __syncthreads();
+    if (q < cnt) {
+        const int t = s_t[q];
+        const Summ<R> r = get(q);
+        tn[t].c = region_record<R>(r);
+        tn[t].l.y = s_skip[q];
+        if (s_ps[q] == -1) { // merged by a higher unit
+            nd[2 * t] = mk4<R>(r.x0, r.y0, r.x1, r.y1);
+            nd[2 * t + 1] = mk4<R>(r.m, r.mx, r.my, (R) 0);
         }
     }
 }
@@ -1351,13 +1385,14 @@ template <typename R> struct EngineImpl final : Engine {
     DevBuf<uint32_t> d_keyA, d_valA, d_keyB, d_valB, d_hist, d_rowtot;
     DevBuf<int2> d_child;
     DevBuf<TNode<R>> d_tn;
-    DevBuf<int> d_parent_leaf, d_parent_int, d_escape, d_visits;
-    DevBuf<int> d_ready; // ready lists: per super-tile at its first position, then the top one
-    DevBuf<int> d_rcnt;  // their lengths (super-tiles, then the top list)
-    DevBuf<Super> d_supers;
-    std::vector<Super> h_supers;
-    int NS = 0;
-    bool multi_block = false; // a graph spans several blocks of k_aggregate_tile
+    DevBuf<int> d_parent_leaf, d_parent_int, d_escape;
+    // summary units of level >= 2, level after level (units of level L at
+    // level_off[L] .. + level_cnt[L]); their region lists and lengths; the
+    // first unit of each (graph, level); each region's level and place
+    DevBuf<Unit> d_units;
+    std::vector<Unit> h_units;
+    int level_off[kMaxLevels] = {0}, level_cnt[kMaxLevels] = {0};
+    DevBuf<int> d_ubase, d_ucnt, d_lists, d_xslot;
     DevBuf<double2> d_part;
 
     // profiling: events between the phases of each iteration
@@ -1487,15 +1522,6 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         D.theta2 = p->theta * p->theta;
         D.jt = p->jitter_tolerance;
         any_adjust = any_adjust || p->adjust;
-        D.pad = (int) h_supers.size(); // first super-tile of the graph
-        for (int s0 = 0; s0 < p->n; s0 += kSuper) {
-            Super S;
-            S.g = g;
-            S.lo = noff + s0;
-            S.hi = std::min(noff + p->n, S.lo + kSuper);
-            S.pad = 0;
-            h_supers.push_back(S);
-        }
         for (int j = 0; j < D.ntiles; j++) {
             Tile t;
             t.g = g;
@@ -1508,11 +1534,21 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         toff += D.ntiles;
     }
     T = toff;
-    NS = (int) h_supers.size();
-    multi_block = false;
-    for (const GraphDesc &D : h_gd) {
-        max_tiles = std::max(max_tiles, D.ntiles);
-        multi_block = multi_block || D.n > agg_block<R>();
+    for (const GraphDesc &D : h_gd) max_tiles = std::max(max_tiles, D.ntiles);
+    // summary units: a graph takes part in level L while it is longer than
+    // a unit of level L - 1; its last level has a single unit
+    std::vector<int> ubase((size_t) G * kMaxLevels, -1);
+    for (int L = 2; L < kMaxLevels; L++) {
+        level_off[L] = (int) h_units.size();
+        const long long below = (long long) agg_block<R>() << (4 * (L - 2)), size = below * kUnitFan;
+        for (int g = 0; g < G; g++) {
+            const long long n = h_gd[g].n;
+            if (n <= below) continue;
+            ubase[(size_t) g * kMaxLevels + L] = (int) h_units.size();
+            for (long long u = 0; u < (n + size - 1) / size; u++)
+                h_units.push_back(Unit{g, (int) h_units.size(), 0, 0});
+        }
+        level_cnt[L] = (int) h_units.size() - level_off[L];
     }
 
     // packed host arrays
@@ -1652,10 +1688,12 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     hs.check(d_parent_leaf.alloc(nn, ast), "alloc");
     hs.check(d_parent_int.alloc(NI, ast), "alloc");
     hs.check(d_escape.alloc(NI, ast), "alloc");
-    hs.check(d_visits.alloc(NI, ast), "alloc");
-    hs.check(d_ready.alloc((size_t) N + NI, ast), "alloc");
-    hs.check(d_rcnt.alloc((size_t) NS + 1, ast), "alloc");
-    hs.check(d_supers.alloc(NS, ast), "alloc");
+    const size_t UT = h_units.size();
+    hs.check(d_units.alloc(UT, ast), "alloc");
+    hs.check(d_ubase.alloc((size_t) G * kMaxLevels, ast), "alloc");
+    hs.check(d_ucnt.alloc(UT * kCountStride, ast), "alloc");
+    hs.check(d_lists.alloc(UT * kUnitCap, ast), "alloc");
+    hs.check(d_xslot.alloc(NI > 0 ? NI : 1, ast), "alloc");
     hs.check(d_part.alloc(T, ast), "alloc");
     if (!hs.ok()) {
         set_hip_error("could not allocate memory on the device", hs.err);
@@ -1670,7 +1708,8 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     up(d_gd.p, h_gd.data(), G * sizeof(GraphDesc));
     up(d_gs.p, gs.data(), G * sizeof(GraphState));
     up(d_tiles.p, h_tiles.data(), T * sizeof(Tile));
-    up(d_supers.p, h_supers.data(), NS * sizeof(Super));
+    up(d_units.p, h_units.data(), h_units.size() * sizeof(Unit));
+    up(d_ubase.p, ubase.data(), ubase.size() * sizeof(int));
     up(d_gid.p, gid.data(), nn * sizeof(int));
     if (NI > 0) up(d_gi.p, gi.data(), (size_t) NI * sizeof(int));
     up(d_pos.p, pos.data(), nn * sizeof(V2));
@@ -1780,7 +1819,8 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
     mark(ev);
     // 1-2: codes
     k_graph_bbox<R><<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_tbox.p, d_gbox.p);
-    k_morton<R><<<T, 256, 0, stream>>>(d_tiles.p, d_pos.p, d_gbox.p, d_keyA.p, d_valA.p);
+    k_morton<R><<<T, 256, 0, stream>>>(d_tiles.p, d_pos.p, d_gbox.p, d_keyA.p, d_valA.p,
+                                       (int) h_units.size(), d_ucnt.p);
     mark(ev);
     // 3: sort, A -> B -> A -> B -> A; the last pass gathers psort
     for (int pass = 0; pass < 4; pass++) {
@@ -1806,21 +1846,16 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
     if (NI > 0) {
         k_karras<R><<<ceil_div(NI, 256), 256, 0, stream>>>(
             NI, d_gi.p, d_gd.p, d_keyA.p, d_child.p, d_tn.p, d_parent_leaf.p, d_parent_int.p,
-            d_escape.p, d_visits.p, NS, d_rcnt.p);
+            d_escape.p, d_ubase.p, d_ucnt.p, d_lists.p, d_xslot.p);
         k_aggregate_tile<R><<<T * (kTile / agg_block<R>()), 1024, 0, stream>>>(
             d_tiles.p, d_gd.p, d_psort.p, d_child.p, d_parent_leaf.p, d_parent_int.p, d_escape.p,
-            NS, N, NI, d_visits.p, d_ready.p, d_rcnt.p, d_nd.p, d_tn.p);
-        if (multi_block) { // some graph spans several blocks of the first level
-            k_aggregate_super<R><<<NS, 1024, 0, stream>>>(d_supers.p, d_gd.p, d_psort.p,
-                                                          d_child.p, d_parent_int.p, d_escape.p,
-                                                          NS, N, NI, d_visits.p, d_ready.p,
-                                                          d_rcnt.p, d_nd.p, d_tn.p);
-            if (NS > G) // and some graph several super-tiles
-                k_aggregate_top<R><<<1, 1024, 0, stream>>>(d_gd.p, d_psort.p, d_child.p,
-                                                            d_parent_int.p, d_escape.p, d_gi.p, NS,
-                                                            N, NI, d_ready.p, d_rcnt.p, d_visits.p,
-                                                            d_nd.p, d_tn.p);
-        }
+            NI, d_nd.p, d_tn.p);
+        for (int L = 2; L < kMaxLevels; L++)
+            if (level_cnt[L] > 0)
+                k_aggregate_unit<R><<<level_cnt[L], 1024, 0, stream>>>(
+                    d_units.p + level_off[L], L, d_gd.p, d_gs.p, d_psort.p, d_child.p,
+                    d_parent_int.p, d_escape.p, d_xslot.p, d_lists.p, d_ucnt.p, NI, d_nd.p,
+                    d_tn.p);
     }
     mark(ev);
 }
@@ -2296,6 +2331,18 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
         enqueue_tree(nullptr);
         err = hipGetLastError();
     }
+    auto best_of_1 = [&](auto &&launch) -> double {
+        float ms = 0.f;
+        if (err == hipSuccess) err = hipEventRecord(e0, stream);
+        if (err == hipSuccess) {
+            launch();
+            err = hipGetLastError();
+        }
+        if (err == hipSuccess) err = hipEventRecord(e1, stream);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        return ms;
+    };
     auto best_of_3 = [&](auto &&launch) -> double {
         double best = 0.0;
         for (int k = 0; k < 3 && err == hipSuccess; k++) {
@@ -2314,21 +2361,31 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
     double t[FA2_TUNE_NTIMES] = {0};
     fa2_tuning pick = shape;
     static const int threads[3] = {64, 128, 256}, unroll[4] = {1, 2, 4, 8};
-    int bk = 0;
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 3; k++)
         t[k] = best_of_3([&] { launch_repulse<false>(Fs, nullptr, threads[k]); });
-        if (t[k] < t[bk]) bk = k;
-    }
+    // each default (256 threads, unroll 4) stays unless another shape is
+    // faster by 2%: closer timings are noise
+    int bk = 2;
+    for (int k = 0; k < 3; k++)
+        if (t[k] < 0.98 * t[bk]) bk = k;
     pick.repulse_threads = threads[bk];
+    // an attraction kernel runs after the traversal, whose records and
+    // nodes fill the caches: each candidate is timed right after one
     for (int c = 0; c < 3; c++) {
         const size_t cnt = c < 2 ? h_class[c].size() : h_items.size();
         if (cnt == 0) continue;
-        int bu = -1;
         for (int k = 0; k < 4; k++) {
-            double &tk = t[3 + 4 * c + k];
-            tk = best_of_3([&] { launch_attract_class(c, unroll[k], Fs); });
-            if (bu < 0 || tk < t[3 + 4 * c + bu]) bu = k;
+            double best = 0.0;
+            for (int rep = 0; rep < 3 && err == hipSuccess; rep++) {
+                launch_repulse<false>(Fs, nullptr);
+                const double tk = best_of_1([&] { launch_attract_class(c, unroll[k], Fs); });
+                if (rep == 0 || tk < best) best = tk;
+            }
+            t[3 + 4 * c + k] = best;
         }
+        int bu = 2;
+        for (int k = 0; k < 4; k++)
+            if (t[3 + 4 * c + k] < 0.98 * t[3 + 4 * c + bu]) bu = k;
         pick.attract_unroll[c] = unroll[bu];
     }
     if (e0) (void) hipEventDestroy(e0);
@@ -2409,15 +2466,18 @@ template <typename R> static size_t footprint_of(long long n, long long incidenc
                        2 * sizeof(double2) /*F*/ + (Acc<R>::separate ? sizeof(double2) : 0) +
                        3 * sizeof(R) /*mass, size*/ + 1 /*fixed*/ + sizeof(int) /*inc_off*/ +
                        sizeof(int4) /*class lists*/ + sizeof(V4) /*psort*/ +
-                       4 * sizeof(uint32_t) /*keys, values*/ + sizeof(int) /*parent_leaf*/ +
-                       sizeof(int) /*ready*/);
+                       4 * sizeof(uint32_t) /*keys, values*/ + sizeof(int) /*parent_leaf*/);
     b += (size_t) ni * (sizeof(int) /*gi*/ + 2 * sizeof(V4) /*nd*/ + sizeof(int2) /*child*/ +
-                        sizeof(TNode<R>) + 4 * sizeof(int) /*parent, escape, visits, ready*/);
+                        sizeof(TNode<R>) + 3 * sizeof(int) /*parent, escape, xslot*/);
+    // summary units of level >= 2: at most n / (15 * block) plus one per level
+    const long long units = n / (15LL * agg_block<R>()) + kMaxLevels;
+    b += (size_t) units * (sizeof(Unit) + (kUnitCap + kCountStride) * sizeof(int));
     b += (size_t) n * sizeof(TNode<R>); // the leaves' traversal records
     b += (size_t) incidences * (sizeof(int) + sizeof(R));
     b += (size_t) items * (2 * sizeof(int4) + sizeof(double2));
     b += (size_t) tiles * (sizeof(Tile) + 2 * sizeof(V4) + sizeof(double2) + 256 * sizeof(uint32_t));
-    b += sizeof(GraphDesc) + sizeof(GraphState) + 256 * sizeof(uint32_t) + 2 * sizeof(Super) + 4096;
+    b += sizeof(GraphDesc) + sizeof(GraphState) + 256 * sizeof(uint32_t) +
+         kMaxLevels * sizeof(int) + 4096;
     return b;
 }
 

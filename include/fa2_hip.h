@@ -164,8 +164,10 @@ typedef struct fa2_tuning {
 
 /* Autotuner: builds the tree of the batch's current positions and times
  * every candidate launch shape on it, with HIP events on the batch's stream
- * (best of three launches each), then keeps the fastest for the following
- * iterations. The layout state is left as it was: the candidates write only
+ * (best of three launches each; an attraction candidate right after a
+ * traversal launch, as in an iteration), then keeps for the following
+ * iterations each default shape (256 threads, unroll 4) unless another was
+ * timed at least 2% faster. The layout state is left as it was: the candidates write only
  * into the force buffer that the next iteration overwrites. Choices are
  * cached per process by device, precision and the batch's size class; with
  * force == 0 a cached choice is reused without timing (times then stay

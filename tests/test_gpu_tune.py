@@ -4,7 +4,8 @@ fa2_batch_set_tuning of include/fa2_hip.h).
 Gates: every candidate shape gives positions bit-identical to the default
 shapes, with the iterations replayed from the captured HIP graph (which is
 recaptured after each change) and launched kernel by kernel; tuning leaves
-the layout state untouched; the tuner keeps the fastest candidate it timed;
+the layout state untouched; the tuner keeps a shape other than the default
+only when it timed it at least 2% faster;
 a server batch above the autotune size gives each graph's layout alone.
 """
 import numpy as np
@@ -57,7 +58,7 @@ def test_every_shape_gives_the_same_layout(precision):
                 assert np.array_equal(out[g], ref[g]), (shape, profile, g)
 
 
-def test_tuning_keeps_the_state_and_the_fastest():
+def test_tuning_keeps_the_state_and_a_faster_shape():
     gs = _graphs()
     ref = _layout(gs, 6)
     with Batch(gs, autotune=False) as b:
@@ -65,11 +66,15 @@ def test_tuning_keeps_the_state_and_the_fastest():
         chosen, times = b.tune(force=True)
         assert times is not None and np.all(times[:3] > 0)
         assert np.all(times[3:] > 0)  # every class is occupied
-        threads = (64, 128, 256)
-        assert chosen["repulse_threads"] == threads[int(np.argmin(times[:3]))]
+        # a shape other than the default (256 threads, unroll 4) is kept only
+        # when it was timed at least 2% faster
+        def kept(cands, values, chosen_value, default):
+            t = dict(zip(cands, values))
+            assert t[chosen_value] <= t[default]
+            assert chosen_value == default or t[chosen_value] < 0.98 * t[default]
+        kept((64, 128, 256), times[:3], chosen["repulse_threads"], 256)
         for c in range(3):
-            u = (1, 2, 4, 8)[int(np.argmin(times[3 + 4 * c:7 + 4 * c]))]
-            assert chosen["attract_unroll"][c] == u
+            kept((1, 2, 4, 8), times[3 + 4 * c:7 + 4 * c], chosen["attract_unroll"][c], 4)
         assert b.tuning == chosen
         b.run(3)
         b.sync()
