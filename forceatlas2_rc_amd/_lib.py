@@ -57,10 +57,11 @@ class Edges(Structure):
 
 class Tuning(Structure):
     """fa2_tuning (include/fa2_hip.h): launch shapes of a batch."""
-    _fields_ = [("attract_unroll", c_int * 3), ("repulse_threads", c_int)]
+    _fields_ = [("attract_unroll", c_int * 3), ("repulse_threads", c_int),
+                ("repulse_xcd_run", c_int)]
 
 
-FA2_TUNE_NTIMES = 15
+FA2_TUNE_NTIMES = 20
 
 
 class FullParams(Structure):

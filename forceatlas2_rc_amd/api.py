@@ -346,7 +346,7 @@ def _fill_params(p, gr, device, precision, profile, edges=None):
 
 def _tuning_dict(t):
     return dict(attract_unroll=tuple(int(t.attract_unroll[c]) for c in range(3)),
-                repulse_threads=int(t.repulse_threads))
+                repulse_threads=int(t.repulse_threads), repulse_xcd_run=int(t.repulse_xcd_run))
 
 
 class Batch:
@@ -403,9 +403,9 @@ class Batch:
     def tune(self, force=True):
         """Time the candidate launch shapes on the current positions and keep
         the fastest (fa2_batch_tune). Returns the chosen shapes and the
-        timings in ms (traversal at 64, 128, 256 threads; then unrolls 1, 2,
-        4, 8 of each attraction class), or None for timings when a cached
-        choice was reused."""
+        timings in ms (traversal at 64, 128, 256 threads; at XCD runs of 0,
+        8, 32, 128, 512 blocks; then unrolls 1, 2, 4, 8 of each attraction class),
+        or None for timings when a cached choice was reused."""
         t = Tuning()
         times = np.full(FA2_TUNE_NTIMES, -1.0)
         if lib.fa2_batch_tune(self.h, int(bool(force)), ctypes.byref(t), _ptr(times)) != 0:
@@ -419,11 +419,12 @@ class Batch:
             raise GPUError(last_error())
         return _tuning_dict(t)
 
-    def set_tuning(self, attract_unroll=(4, 4, 4), repulse_threads=256):
+    def set_tuning(self, attract_unroll=(4, 4, 4), repulse_threads=256, repulse_xcd_run=0):
         t = Tuning()
         for c in range(3):
             t.attract_unroll[c] = int(attract_unroll[c])
         t.repulse_threads = int(repulse_threads)
+        t.repulse_xcd_run = int(repulse_xcd_run)
         if lib.fa2_batch_set_tuning(self.h, ctypes.byref(t)) != 0:
             raise GPUError(last_error())
 

@@ -801,6 +801,21 @@ k_aggregate_unit(const Unit *__restrict__ units, int level, const GraphDesc *__r
     }
 }
 
+// Workgroup b of a launch runs on XCD b mod 8, each XCD with its own L2.
+// With run > 0, workgroup b takes the block that gives each XCD runs of
+// `run` consecutive blocks (a bijection of [0, nb)): neighbouring nodes walk
+// mostly the same tree records, which one L2 then holds for the whole run.
+// Which hardware runs a block changes nothing in what the block computes.
+__device__ inline int xcd_block(int b, int nb, int run)
+{
+    constexpr int kXcds = 8;
+    if (run <= 0) return b;
+    const int span = kXcds * run, full = nb / span * span;
+    if (b >= full) return b;
+    const int within = b % span;
+    return b - within + (within % kXcds) * run + within / kXcds;
+}
+
 // ---------------------------------------------------------------------------
 // Repulsion: Barnes-Hut over the Karras tree (pipeline.cl:332-383), walked
 // by a whole wave at once.
@@ -864,11 +879,11 @@ __global__ void __launch_bounds__(256)
 k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
           const GraphState *__restrict__ gs, const typename Vec<R>::v4 *__restrict__ psort,
           const TNode<R> *__restrict__ tn, const uint32_t *__restrict__ order,
-          double2 *__restrict__ F, unsigned long long *cnt = nullptr)
+          double2 *__restrict__ F, int xrun, unsigned long long *cnt = nullptr)
 {
     using V4 = typename Vec<R>::v4;
     unsigned long long nA = 0, nO = 0, nP = 0, nS = 0;
-    const int k0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k0 = xcd_block((int) blockIdx.x, (int) gridDim.x, xrun) * blockDim.x + threadIdx.x;
     const int k = k0 < N ? k0 : N - 1;
     const bool valid = k0 < N && !gs[gid[k]].retired;
     const int my_g = valid ? gid[k] : -1;
@@ -1405,7 +1420,7 @@ template <typename R> struct EngineImpl final : Engine {
     hipGraphExec_t graph_exec = nullptr;
 
     // launch shapes (none changes a result)
-    fa2_tuning shape = {{4, 4, 4}, 256};
+    fa2_tuning shape = {{4, 4, 4}, 256, 0};
 
     ~EngineImpl() override
     {
@@ -1449,17 +1464,19 @@ template <typename R> struct EngineImpl final : Engine {
 
     // the repulsion kernel for this batch: with sizes or not, counting or
     // not, record offsets in 32 or 64 bits; `threads` per workgroup (64,
-    // 128 or 256; each wave walks for the same 64 nodes whatever the size)
+    // 128 or 256) and XCD runs of `xrun` blocks (0: none); each wave walks
+    // for the same 64 nodes whatever the shape
     template <bool COUNT> void launch_repulse(double2 *Fout, unsigned long long *cnt,
-                                              int threads = 0)
+                                              int threads = 0, int xrun = -1)
     {
         const bool wide = ((size_t) NI + (size_t) N) * sizeof(TNode<R>) >= ((size_t) 1 << 32);
         if (threads <= 0) threads = shape.repulse_threads;
+        if (xrun < 0) xrun = shape.repulse_xcd_run;
         const unsigned blocks = ceil_div(N, threads);
 #define FA2_REPULSE(ADJ, WIDE)                                                                   \
     k_repulse<R, ADJ, COUNT, WIDE><<<blocks, threads, 0, stream>>>(N, NI, d_gid.p, d_gd.p, d_gs.p, \
                                                                    d_psort.p, d_tn.p, d_valA.p,    \
-                                                                   Fout, cnt)
+                                                                   Fout, xrun, cnt)
         if (any_adjust) {
             if (wide) FA2_REPULSE(true, true);
             else FA2_REPULSE(true, false);
@@ -2299,6 +2316,10 @@ template <typename R> int EngineImpl<R>::set_tuning(const fa2_tuning *t)
         set_message("traversal workgroup of %d threads: 64, 128 or 256", t->repulse_threads);
         return 1;
     }
+    if (t->repulse_xcd_run < 0 || t->repulse_xcd_run > 4096) {
+        set_message("XCD run of %d blocks: 0 to 4096", t->repulse_xcd_run);
+        return 1;
+    }
     (void) hipSetDevice(device);
     (void) hipStreamSynchronize(stream); // a captured graph may still be running
     shape = *t;
@@ -2362,13 +2383,21 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
     fa2_tuning pick = shape;
     static const int threads[3] = {64, 128, 256}, unroll[4] = {1, 2, 4, 8};
     for (int k = 0; k < 3; k++)
-        t[k] = best_of_3([&] { launch_repulse<false>(Fs, nullptr, threads[k]); });
-    // each default (256 threads, unroll 4) stays unless another shape is
-    // faster by 2%: closer timings are noise
+        t[k] = best_of_3([&] { launch_repulse<false>(Fs, nullptr, threads[k], 0); });
+    // each default (256 threads, no XCD runs, unroll 4) stays unless another
+    // shape is faster by 2%: closer timings are noise
     int bk = 2;
     for (int k = 0; k < 3; k++)
         if (t[k] < 0.98 * t[bk]) bk = k;
     pick.repulse_threads = threads[bk];
+    static const int runs[5] = {0, 8, 32, 128, 512};
+    for (int k = 0; k < 5; k++)
+        t[3 + k] = best_of_3(
+            [&] { launch_repulse<false>(Fs, nullptr, pick.repulse_threads, runs[k]); });
+    int bx = 0;
+    for (int k = 0; k < 5; k++)
+        if (t[3 + k] < 0.98 * t[3 + bx]) bx = k;
+    pick.repulse_xcd_run = runs[bx];
     // an attraction kernel runs after the traversal, whose records and
     // nodes fill the caches: each candidate is timed right after one
     for (int c = 0; c < 3; c++) {
@@ -2377,15 +2406,15 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
         for (int k = 0; k < 4; k++) {
             double best = 0.0;
             for (int rep = 0; rep < 3 && err == hipSuccess; rep++) {
-                launch_repulse<false>(Fs, nullptr);
+                launch_repulse<false>(Fs, nullptr, pick.repulse_threads, pick.repulse_xcd_run);
                 const double tk = best_of_1([&] { launch_attract_class(c, unroll[k], Fs); });
                 if (rep == 0 || tk < best) best = tk;
             }
-            t[3 + 4 * c + k] = best;
+            t[8 + 4 * c + k] = best;
         }
         int bu = 2;
         for (int k = 0; k < 4; k++)
-            if (t[3 + 4 * c + k] < 0.98 * t[3 + 4 * c + bu]) bu = k;
+            if (t[8 + 4 * c + k] < 0.98 * t[8 + 4 * c + bu]) bu = k;
         pick.attract_unroll[c] = unroll[bu];
     }
     if (e0) (void) hipEventDestroy(e0);

@@ -148,26 +148,31 @@ int fa2_batch_set_state(fa2_batch *b, int g, const double *pos, const double *f_
  * the speed: attract_unroll[c] is the number of incidences whose loads each
  * lane has in flight at once (1, 2, 4 or 8) in the attraction kernel of
  * class c (c = 0: nodes of 1..16 incidences, one lane each; 1: 17..256,
- * 16 lanes each; 2: longer lists, one wave per chunk), and
+ * 16 lanes each; 2: longer lists, one wave per chunk),
  * repulse_threads the workgroup size of the Barnes-Hut traversal (64, 128
- * or 256). The reference has no counterpart (its work sizes are fixed in
+ * or 256) and repulse_xcd_run its mapping of blocks to XCDs. The reference has no counterpart (its work sizes are fixed in
  * src/gpu_full.c:193-198). */
 typedef struct fa2_tuning {
     int attract_unroll[3];
     int repulse_threads;
+    /* 0, or the traversal's workgroups handed to each XCD in runs of this
+     * many consecutive blocks (0..4096), so that one L2 serves the tree
+     * records of neighbouring nodes */
+    int repulse_xcd_run;
 } fa2_tuning;
 
 /* Timings behind fa2_batch_tune: the traversal at 64, 128 and 256 threads,
- * then for each attraction class the unrolls 1, 2, 4 and 8 (ms; 0 where a
- * class is empty). */
-#define FA2_TUNE_NTIMES 15
+ * then at XCD runs of 0, 8, 32, 128 and 512 blocks, then for each
+ * attraction class the unrolls 1, 2, 4 and 8 (ms; 0 where a class is
+ * empty). */
+#define FA2_TUNE_NTIMES 20
 
 /* Autotuner: builds the tree of the batch's current positions and times
  * every candidate launch shape on it, with HIP events on the batch's stream
  * (best of three launches each; an attraction candidate right after a
  * traversal launch, as in an iteration), then keeps for the following
- * iterations each default shape (256 threads, unroll 4) unless another was
- * timed at least 2% faster. The layout state is left as it was: the candidates write only
+ * iterations each default shape (256 threads, no XCD runs, unroll 4) unless
+ * another was timed at least 2% faster. The layout state is left as it was: the candidates write only
  * into the force buffer that the next iteration overwrites. Choices are
  * cached per process by device, precision and the batch's size class; with
  * force == 0 a cached choice is reused without timing (times then stay

@@ -15,8 +15,8 @@ from forceatlas2_rc_amd import Batch, GPUError, Server, graphs
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [((1, 1, 1), 64), ((2, 2, 2), 128), ((8, 8, 8), 256), ((4, 1, 8), 64),
-          ((8, 4, 2), 128)]
+SHAPES = [((1, 1, 1), 64, 0), ((2, 2, 2), 128, 2), ((8, 8, 8), 256, 32), ((4, 1, 8), 64, 8),
+          ((8, 4, 2), 128, 1), ((4, 4, 4), 256, 4096)]
 
 
 def _graphs():
@@ -73,8 +73,9 @@ def test_tuning_keeps_the_state_and_a_faster_shape():
             assert t[chosen_value] <= t[default]
             assert chosen_value == default or t[chosen_value] < 0.98 * t[default]
         kept((64, 128, 256), times[:3], chosen["repulse_threads"], 256)
+        kept((0, 8, 32, 128, 512), times[3:8], chosen["repulse_xcd_run"], 0)
         for c in range(3):
-            kept((1, 2, 4, 8), times[3 + 4 * c:7 + 4 * c], chosen["attract_unroll"][c], 4)
+            kept((1, 2, 4, 8), times[8 + 4 * c:12 + 4 * c], chosen["attract_unroll"][c], 4)
         assert b.tuning == chosen
         b.run(3)
         b.sync()
@@ -90,10 +91,11 @@ def test_tuning_keeps_the_state_and_a_faster_shape():
 def test_tuning_errors():
     gs = _graphs()[1:2]
     with Batch(gs, autotune=False) as b:
-        for bad in (((3, 4, 4), 256), ((4, 4, 4), 96), ((4, 0, 4), 64)):
+        for bad in (((3, 4, 4), 256, 0), ((4, 4, 4), 96, 0), ((4, 0, 4), 64, 0),
+                    ((4, 4, 4), 256, -1), ((4, 4, 4), 256, 5000)):
             with pytest.raises(GPUError):
                 b.set_tuning(*bad)
-        assert b.tuning == dict(attract_unroll=(4, 4, 4), repulse_threads=256)
+        assert b.tuning == dict(attract_unroll=(4, 4, 4), repulse_threads=256, repulse_xcd_run=0)
 
 
 def test_server_batches_above_the_autotune_size():
