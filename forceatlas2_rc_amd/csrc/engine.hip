@@ -1111,7 +1111,7 @@ __global__ void __launch_bounds__(256)
 k_attract(int count, const int4 *__restrict__ work, const GraphDesc *__restrict__ gd,
           const GraphState *__restrict__ gs, const typename Vec<R>::v2 *__restrict__ pos,
           const R *__restrict__ size, const int *__restrict__ inc_node,
-          const R *__restrict__ inc_coef, double2 *__restrict__ F)
+          const R *__restrict__ inc_coef, double2 *__restrict__ Fa)
 {
     const int slot = blockIdx.x * (256 / GL) + (int) threadIdx.x / GL;
     const int sub = (int) threadIdx.x % GL;
@@ -1124,10 +1124,7 @@ k_attract(int count, const int4 *__restrict__ work, const GraphDesc *__restrict_
                         size[i], pos, size, inc_node, inc_coef, sum);
     group_sum<GL>(sum.x, sum.y);
     if (!live || sub != 0) return;
-    double2 f = F[i];
-    f.x += sum.x;
-    f.y += sum.y;
-    F[i] = f;
+    Fa[i] = make_double2(sum.x, sum.y);
 }
 
 // One wave per item (node, first incidence, end, partial slot or -1).
@@ -1137,7 +1134,7 @@ k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict
                 const GraphDesc *__restrict__ gd, const GraphState *__restrict__ gs,
                 const typename Vec<R>::v2 *__restrict__ pos,
                 const R *__restrict__ size, const int *__restrict__ inc_node,
-                const R *__restrict__ inc_coef, double2 *__restrict__ F,
+                const R *__restrict__ inc_coef, double2 *__restrict__ Fa,
                 double2 *__restrict__ part)
 {
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1153,10 +1150,7 @@ k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict
     group_sum<64>(sum.x, sum.y);
     if (lane != 0 || gs[g].retired) return;
     if (it.w < 0) {
-        double2 f = F[i];
-        f.x += sum.x;
-        f.y += sum.y;
-        F[i] = f;
+        Fa[i] = make_double2(sum.x, sum.y);
     } else {
         part[it.w] = make_double2(sum.x, sum.y);
     }
@@ -1167,7 +1161,7 @@ template <typename R>
 __global__ void __launch_bounds__(256)
 k_attract_fix(int count, const int4 *__restrict__ hubs, const int *__restrict__ gid,
               const GraphState *__restrict__ gs, const double2 *__restrict__ part,
-              double2 *__restrict__ F)
+              double2 *__restrict__ Fa)
 {
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= count) return;
@@ -1182,10 +1176,7 @@ k_attract_fix(int count, const int4 *__restrict__ hubs, const int *__restrict__ 
     }
     group_sum<64>(ax, ay);
     if (lane != 0) return;
-    double2 f = F[h.x];
-    f.x += ax;
-    f.y += ay;
-    F[h.x] = f;
+    Fa[h.x] = make_double2(ax, ay);
 }
 
 // ---------------------------------------------------------------------------
@@ -1206,17 +1197,29 @@ __device__ inline void swing_node(double2 f, double2 fo, double m, double &sw, d
 template <typename R>
 __global__ void __launch_bounds__(256)
 k_swing(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
-        const R *__restrict__ mass, const uint8_t *__restrict__ fixed,
-        const double2 *__restrict__ F, const double2 *__restrict__ Fold,
+        const GraphState *__restrict__ gs, const R *__restrict__ mass,
+        const uint8_t *__restrict__ fixed, double2 *__restrict__ F,
+        const double2 *__restrict__ Fa, const double2 *__restrict__ Fold,
         double2 *__restrict__ part)
 {
     __shared__ double red[2][4];
     const Tile t = tiles[blockIdx.x];
     const bool use_fixed = gd[t.g].flags & GF_FIXED;
+    // the force of the iteration: repulsion minus gravity, plus the
+    // attraction (held apart while the two were computed side by side;
+    // zero for nodes without edges)
+    const bool live = !gs[t.g].retired;
     double a = 0.0, b = 0.0;
     for (int i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
+        double2 f = F[i];
+        if (live) {
+            const double2 fa = Fa[i];
+            f.x += fa.x;
+            f.y += fa.y;
+            F[i] = f;
+        }
         double sw, tr;
-        swing_node(F[i], Fold[i], (double) mass[i], sw, tr);
+        swing_node(f, Fold[i], (double) mass[i], sw, tr);
         if (use_fixed && fixed[i]) continue;
         a += sw;
         b += tr;
@@ -1389,6 +1392,11 @@ template <typename R> struct EngineImpl final : Engine {
     DevBuf<V4> d_node4; // (x, y, mass, radius) in the original order
     int max_tiles = 1;
     DevBuf<double2> d_F[2]; // forces are kept in FP64: the swing is their difference
+    // the attraction of the iteration, computed beside the repulsion on a
+    // second stream and added to it by k_swing (zero for nodes without edges)
+    DevBuf<double2> d_Fa;
+    hipStream_t s_att = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DevBuf<double2> d_acc;
     DevBuf<R> d_mass, d_size, d_coef;
     DevBuf<uint8_t> d_fixed;
@@ -1424,10 +1432,14 @@ template <typename R> struct EngineImpl final : Engine {
 
     ~EngineImpl() override
     {
+        if (s_att) (void) hipStreamSynchronize(s_att);
         if (stream) (void) hipStreamSynchronize(stream);
         if (graph_exec) (void) hipGraphExecDestroy(graph_exec);
         if (graph) (void) hipGraphDestroy(graph);
         for (hipEvent_t e : ev_pool) (void) hipEventDestroy(e);
+        if (ev_fork) (void) hipEventDestroy(ev_fork);
+        if (ev_join) (void) hipEventDestroy(ev_join);
+        if (s_att) (void) hipStreamDestroy(s_att);
     }
 
     int open(const EngineGraph *gr, int ngraphs) override;
@@ -1458,8 +1470,10 @@ template <typename R> struct EngineImpl final : Engine {
     // phases 1-4: codes, sort, tree of the current positions (scratch
     // buffers only)
     void enqueue_tree(std::vector<hipEvent_t> *ev);
-    // the attraction kernel of class c at unroll u, adding into Fc
-    void launch_attract_class(int c, int u, double2 *Fc);
+    // the attraction kernel of class c at unroll u, writing into Fa
+    void launch_attract_class(int c, int u, double2 *Fa, hipStream_t st);
+    // every attraction kernel of an iteration, on st
+    void launch_attraction(hipStream_t st);
     void drop_graph();
 
     // the repulsion kernel for this batch: with sizes or not, counting or
@@ -1677,6 +1691,7 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     hs.check(d_node4.alloc(nn, ast), "alloc");
     hs.check(d_F[0].alloc(nn, ast), "alloc");
     hs.check(d_F[1].alloc(nn, ast), "alloc");
+    hs.check(d_Fa.alloc(nn, ast), "alloc");
     hs.check(d_acc.alloc(Acc<R>::separate ? nn : 1, ast), "alloc");
     if (!from_edges) {
         hs.check(d_mass.alloc(nn, ast), "alloc");
@@ -1755,6 +1770,12 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         hs.check(hipMemsetAsync(d_psort.p + nn, 0, kLeafSize * sizeof(V4), stream), "memset");
     if (hs.ok()) hs.check(hipMemsetAsync(d_F[0].p, 0, nn * sizeof(double2), stream), "memset");
     if (hs.ok()) hs.check(hipMemsetAsync(d_F[1].p, 0, nn * sizeof(double2), stream), "memset");
+    if (hs.ok()) hs.check(hipMemsetAsync(d_Fa.p, 0, nn * sizeof(double2), stream), "memset");
+    // the attraction's stream (default priority) and the events that fork it
+    // from the batch's stream and join it back
+    if (hs.ok() && !s_att) hs.check(hipStreamCreateWithFlags(&s_att, hipStreamNonBlocking), "stream");
+    if (hs.ok() && !ev_fork) hs.check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "event");
+    if (hs.ok() && !ev_join) hs.check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "event");
     if (hs.ok()) {
         k_tile_bbox<R><<<T, 256, 0, stream>>>(d_tiles.p, d_pos.p, d_tbox.p);
         hs.check(hipGetLastError(), "launch");
@@ -1783,22 +1804,23 @@ template <typename R> void EngineImpl<R>::mark(std::vector<hipEvent_t> *ev)
     ev->push_back(e);
 }
 
-template <typename R> void EngineImpl<R>::launch_attract_class(int c, int u, double2 *Fc)
+template <typename R>
+void EngineImpl<R>::launch_attract_class(int c, int u, double2 *Fa, hipStream_t st)
 {
     const int cnt = c < 2 ? (int) h_class[c].size() : (int) h_items.size();
     if (cnt == 0) return;
 #define FA2_ATTRACT(U)                                                                          \
     do {                                                                                        \
         if (c == 0)                                                                             \
-            k_attract<R, 1, U><<<ceil_div(cnt, 256), 256, 0, stream>>>(                         \
-                cnt, d_class[0].p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_node.p, d_coef.p, Fc); \
+            k_attract<R, 1, U><<<ceil_div(cnt, 256), 256, 0, st>>>(                         \
+                cnt, d_class[0].p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_node.p, d_coef.p, Fa); \
         else if (c == 1)                                                                        \
-            k_attract<R, 16, U><<<ceil_div(cnt, 16), 256, 0, stream>>>(                         \
-                cnt, d_class[1].p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_node.p, d_coef.p, Fc); \
+            k_attract<R, 16, U><<<ceil_div(cnt, 16), 256, 0, st>>>(                         \
+                cnt, d_class[1].p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_node.p, d_coef.p, Fa); \
         else                                                                                    \
-            k_attract_items<R, U><<<ceil_div(cnt, 4), 256, 0, stream>>>(                        \
+            k_attract_items<R, U><<<ceil_div(cnt, 4), 256, 0, st>>>(                        \
                 cnt, d_items.p, d_gid.p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_node.p,        \
-                d_coef.p, Fc, d_hubpart.p);                                                     \
+                d_coef.p, Fa, d_hubpart.p);                                                     \
     } while (0)
     switch (u) {
     case 1: FA2_ATTRACT(1); break;
@@ -1809,21 +1831,40 @@ template <typename R> void EngineImpl<R>::launch_attract_class(int c, int u, dou
 #undef FA2_ATTRACT
 }
 
+template <typename R> void EngineImpl<R>::launch_attraction(hipStream_t st)
+{
+    for (int c = 0; c < 3; c++) launch_attract_class(c, shape.attract_unroll[c], d_Fa.p, st);
+    if (!h_hubs.empty())
+        k_attract_fix<R><<<ceil_div((int) h_hubs.size(), 4), 256, 0, st>>>(
+            (int) h_hubs.size(), d_hubs.p, d_gid.p, d_gs.p, d_hubpart.p, d_Fa.p);
+}
+
 template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent_t> *ev)
 {
     double2 *Fc = d_F[parity].p, *Fo = d_F[1 - parity].p;
+    // 6: the attraction needs only the positions of the last displacement.
+    // Without profiling it runs on the second stream, forked here (after the
+    // last displacement) and joined before the swing, beside phases 1-5,
+    // which are bound by other units (scalar loads and VALU, not the
+    // vector memory pipeline of its gathers).
+    const bool side = !ev && s_att;
+    if (side) {
+        (void) hipEventRecord(ev_fork, stream);
+        (void) hipStreamWaitEvent(s_att, ev_fork, 0);
+        launch_attraction(s_att);
+        (void) hipEventRecord(ev_join, s_att);
+    }
     enqueue_tree(ev);
-    // 5: repulsion
+    // 5: repulsion minus gravity
     launch_repulse<false>(Fc, nullptr);
     mark(ev);
-    // 6: attraction (gravity was applied with the repulsion)
-    for (int c = 0; c < 3; c++) launch_attract_class(c, shape.attract_unroll[c], Fc);
-    if (!h_hubs.empty())
-        k_attract_fix<R><<<ceil_div((int) h_hubs.size(), 4), 256, 0, stream>>>(
-            (int) h_hubs.size(), d_hubs.p, d_gid.p, d_gs.p, d_hubpart.p, Fc);
+    if (side) (void) hipStreamWaitEvent(stream, ev_join, 0);
+    else launch_attraction(stream);
     mark(ev);
-    // 7-8: swinging, speed, displacement
-    k_swing<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_mass.p, d_fixed.p, Fc, Fo, d_part.p);
+    // 7-8: the force (k_swing adds the attraction), swinging, speed,
+    // displacement
+    k_swing<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, d_mass.p, d_fixed.p, Fc, d_Fa.p,
+                                      Fo, d_part.p);
     k_speed<<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_part.p, d_gs.p);
     k_apply<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, d_mass.p, d_fixed.p, Fc, Fo,
                                       d_pos.p, d_size.p, d_node4.p, d_acc.p, d_tbox.p);
@@ -2386,18 +2427,19 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
         t[k] = best_of_3([&] { launch_repulse<false>(Fs, nullptr, threads[k], 0); });
     // each default (256 threads, no XCD runs, unroll 4) stays unless another
     // shape is faster by 2%: closer timings are noise
-    int bk = 2;
-    for (int k = 0; k < 3; k++)
-        if (t[k] < 0.98 * t[bk]) bk = k;
-    pick.repulse_threads = threads[bk];
+    // the fastest candidate replaces the default when 2% faster
+    auto choose = [&](const double *tt, int n, int dflt) {
+        int b = 0;
+        for (int k = 1; k < n; k++)
+            if (tt[k] < tt[b]) b = k;
+        return tt[b] < 0.98 * tt[dflt] ? b : dflt;
+    };
+    pick.repulse_threads = threads[choose(t, 3, 2)];
     static const int runs[5] = {0, 8, 32, 128, 512};
     for (int k = 0; k < 5; k++)
         t[3 + k] = best_of_3(
             [&] { launch_repulse<false>(Fs, nullptr, pick.repulse_threads, runs[k]); });
-    int bx = 0;
-    for (int k = 0; k < 5; k++)
-        if (t[3 + k] < 0.98 * t[3 + bx]) bx = k;
-    pick.repulse_xcd_run = runs[bx];
+    pick.repulse_xcd_run = runs[choose(t + 3, 5, 0)];
     // an attraction kernel runs after the traversal, whose records and
     // nodes fill the caches: each candidate is timed right after one
     for (int c = 0; c < 3; c++) {
@@ -2407,15 +2449,13 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
             double best = 0.0;
             for (int rep = 0; rep < 3 && err == hipSuccess; rep++) {
                 launch_repulse<false>(Fs, nullptr, pick.repulse_threads, pick.repulse_xcd_run);
-                const double tk = best_of_1([&] { launch_attract_class(c, unroll[k], Fs); });
+                const double tk =
+                    best_of_1([&] { launch_attract_class(c, unroll[k], d_Fa.p, stream); });
                 if (rep == 0 || tk < best) best = tk;
             }
             t[8 + 4 * c + k] = best;
         }
-        int bu = 2;
-        for (int k = 0; k < 4; k++)
-            if (t[8 + 4 * c + k] < 0.98 * t[8 + 4 * c + bu]) bu = k;
-        pick.attract_unroll[c] = unroll[bu];
+        pick.attract_unroll[c] = unroll[choose(t + 8 + 4 * c, 4, 2)];
     }
     if (e0) (void) hipEventDestroy(e0);
     if (e1) (void) hipEventDestroy(e1);
@@ -2492,7 +2532,7 @@ template <typename R> static size_t footprint_of(long long n, long long incidenc
     const long long items = incidences / kChunk + n / 256 + 2;
     size_t b = 0;
     b += (size_t) n * (sizeof(int) /*gid*/ + sizeof(V2) /*pos*/ + sizeof(V4) /*node4*/ +
-                       2 * sizeof(double2) /*F*/ + (Acc<R>::separate ? sizeof(double2) : 0) +
+                       3 * sizeof(double2) /*F, Fa*/ + (Acc<R>::separate ? sizeof(double2) : 0) +
                        3 * sizeof(R) /*mass, size*/ + 1 /*fixed*/ + sizeof(int) /*inc_off*/ +
                        sizeof(int4) /*class lists*/ + sizeof(V4) /*psort*/ +
                        4 * sizeof(uint32_t) /*keys, values*/ + sizeof(int) /*parent_leaf*/);
