@@ -194,6 +194,7 @@ def batched_leg(args):
     t0 = time.perf_counter()
     b = Batch(gs)
     t_up = time.perf_counter()
+    t_open = t_up - t0
     b.run(it)
     b.sync()
     t_run = time.perf_counter()
@@ -214,7 +215,7 @@ def batched_leg(args):
     return dict(workload=(f"{len(gs)} SBM graphs, 200..20000 nodes log-uniform, 2..20 blocks, "
                           f"in-block degree 8, cross-block 1, {it} iterations each, BH theta=1.2"),
                 graphs=len(gs), nodes=n_tot, edges=m_tot, iterations=it,
-                batched_s=t_batch, open_s=t_up - t0,
+                batched_s=t_batch, open_s=t_open,
                 batched_ms_per_iteration=(t_run - t_up) / it * 1e3,
                 one_at_a_time_s=t_one, layouts_per_s_batched=len(gs) / t_batch,
                 layouts_per_s_one_at_a_time=len(gs) / t_one, speedup=t_one / t_batch,

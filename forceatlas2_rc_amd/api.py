@@ -419,7 +419,7 @@ class Batch:
             raise GPUError(last_error())
         return _tuning_dict(t)
 
-    def set_tuning(self, attract_unroll=(4, 4, 4), repulse_threads=256, repulse_xcd_run=0):
+    def set_tuning(self, attract_unroll=(4, 4, 4), repulse_threads=256, repulse_xcd_run=32):
         t = Tuning()
         for c in range(3):
             t.attract_unroll[c] = int(attract_unroll[c])

@@ -30,6 +30,9 @@ struct Engine {
     int async_alloc = 0;
     // replay the iterations from a captured HIP graph (without profile)
     int use_graph = 1;
+    // compute the attraction on a second stream beside the tree and the
+    // repulsion (without profile)
+    int overlap = 1;
     virtual ~Engine() = default;
     // Upload the graphs; 0 on success (message in fa2_gpu_error()).
     virtual int open(const EngineGraph *graphs, int ngraphs) = 0;

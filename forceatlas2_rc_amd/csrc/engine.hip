@@ -1428,7 +1428,7 @@ template <typename R> struct EngineImpl final : Engine {
     hipGraphExec_t graph_exec = nullptr;
 
     // launch shapes (none changes a result)
-    fa2_tuning shape = {{4, 4, 4}, 256, 0};
+    fa2_tuning shape = {{4, 4, 4}, 256, 32};
 
     ~EngineImpl() override
     {
@@ -1847,7 +1847,7 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
     // last displacement) and joined before the swing, beside phases 1-5,
     // which are bound by other units (scalar loads and VALU, not the
     // vector memory pipeline of its gathers).
-    const bool side = !ev && s_att;
+    const bool side = !ev && s_att && overlap;
     if (side) {
         (void) hipEventRecord(ev_fork, stream);
         (void) hipStreamWaitEvent(s_att, ev_fork, 0);
@@ -2425,8 +2425,8 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
     static const int threads[3] = {64, 128, 256}, unroll[4] = {1, 2, 4, 8};
     for (int k = 0; k < 3; k++)
         t[k] = best_of_3([&] { launch_repulse<false>(Fs, nullptr, threads[k], 0); });
-    // each default (256 threads, no XCD runs, unroll 4) stays unless another
-    // shape is faster by 2%: closer timings are noise
+    // each default (256 threads, XCD runs of 32 blocks, unroll 4) stays
+    // unless another shape is faster by 2%: closer timings are noise
     // the fastest candidate replaces the default when 2% faster
     auto choose = [&](const double *tt, int n, int dflt) {
         int b = 0;
@@ -2439,7 +2439,7 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
     for (int k = 0; k < 5; k++)
         t[3 + k] = best_of_3(
             [&] { launch_repulse<false>(Fs, nullptr, pick.repulse_threads, runs[k]); });
-    pick.repulse_xcd_run = runs[choose(t + 3, 5, 0)];
+    pick.repulse_xcd_run = runs[choose(t + 3, 5, 2)];
     // an attraction kernel runs after the traversal, whose records and
     // nodes fill the caches: each candidate is timed right after one
     for (int c = 0; c < 3; c++) {

@@ -42,10 +42,6 @@ namespace {
 
 enum { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3 };
 
-// batches of at least this many nodes take their launch shapes from the
-// autotuner
-constexpr long long kAutotuneNodes = 65536;
-
 struct Job {
     int id = 0;
     fa2_full_params p{};
@@ -155,6 +151,11 @@ bool fa2_server::admit(std::unique_lock<std::mutex> &lk)
         r->e->device = device;
         r->e->stream = st;
         r->e->async_alloc = 1;
+        // batches already run side by side: one stream each (a second one
+        // per batch would share the hardware queues between batches), and the
+        // default launch shapes (the autotuner synchronises the device and
+        // would hold the worker between chunks)
+        r->e->overlap = 0;
         std::vector<EngineGraph> gr(take.size());
         for (size_t k = 0; k < take.size(); k++)
             gr[k] = EngineGraph{&ps[k], pos[k], has_edges ? &take[k]->edges : nullptr};
@@ -162,12 +163,6 @@ bool fa2_server::admit(std::unique_lock<std::mutex> &lk)
         else if (hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
             err = "could not create a HIP event";
     }
-    // large batches: launch shapes from the autotuner (cached per size class,
-    // so similar batches reuse the first one's timings)
-    long long nodes = 0;
-    for (const fa2_full_params &q : ps) nodes += q.n;
-    if (err.empty() && nodes >= kAutotuneNodes && r->e->tune(0, nullptr, nullptr) != 0)
-        err = fa2_gpu_error();
     if (err.empty()) {
         r->target = std::min(chunk, r->longest);
         if (r->e->run(r->target) != 0 || hipEventRecord(r->ev, st) != hipSuccess)

@@ -156,8 +156,8 @@ typedef struct fa2_tuning {
     int attract_unroll[3];
     int repulse_threads;
     /* 0, or the traversal's workgroups handed to each XCD in runs of this
-     * many consecutive blocks (0..4096), so that one L2 serves the tree
-     * records of neighbouring nodes */
+     * many consecutive blocks (0..4096; default 32), so that one L2 serves
+     * the tree records of neighbouring nodes */
     int repulse_xcd_run;
 } fa2_tuning;
 
@@ -171,7 +171,7 @@ typedef struct fa2_tuning {
  * every candidate launch shape on it, with HIP events on the batch's stream
  * (best of three launches each; an attraction candidate right after a
  * traversal launch, as in an iteration), then keeps for the following
- * iterations each default shape (256 threads, no XCD runs, unroll 4) unless
+ * iterations each default shape (256 threads, XCD runs of 32, unroll 4) unless
  * another was timed at least 2% faster. The layout state is left as it was: the candidates write only
  * into the force buffer that the next iteration overwrites. Choices are
  * cached per process by device, precision and the batch's size class; with

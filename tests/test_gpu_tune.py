@@ -6,7 +6,8 @@ shapes, with the iterations replayed from the captured HIP graph (which is
 recaptured after each change) and launched kernel by kernel; tuning leaves
 the layout state untouched; the tuner keeps a shape other than the default
 only when it timed it at least 2% faster;
-a server batch above the autotune size gives each graph's layout alone.
+a large server batch (default shapes, no second stream) gives each
+graph's layout alone.
 """
 import numpy as np
 import pytest
@@ -66,15 +67,15 @@ def test_tuning_keeps_the_state_and_a_faster_shape():
         chosen, times = b.tune(force=True)
         assert times is not None and np.all(times[:3] > 0)
         assert np.all(times[3:] > 0)  # every class is occupied
-        # the fastest candidate replaces the default (256 threads, no XCD
-        # runs, unroll 4) when it was timed at least 2% faster
+        # the fastest candidate replaces the default (256 threads, XCD runs
+        # of 32 blocks, unroll 4) when it was timed at least 2% faster
         def kept(cands, values, chosen_value, default):
             t = dict(zip(cands, values))
             fastest = min(cands, key=lambda c: t[c])
             want = fastest if t[fastest] < 0.98 * t[default] else default
             assert chosen_value == want
         kept((64, 128, 256), times[:3], chosen["repulse_threads"], 256)
-        kept((0, 8, 32, 128, 512), times[3:8], chosen["repulse_xcd_run"], 0)
+        kept((0, 8, 32, 128, 512), times[3:8], chosen["repulse_xcd_run"], 32)
         for c in range(3):
             kept((1, 2, 4, 8), times[8 + 4 * c:12 + 4 * c], chosen["attract_unroll"][c], 4)
         assert b.tuning == chosen
@@ -96,10 +97,10 @@ def test_tuning_errors():
                     ((4, 4, 4), 256, -1), ((4, 4, 4), 256, 5000)):
             with pytest.raises(GPUError):
                 b.set_tuning(*bad)
-        assert b.tuning == dict(attract_unroll=(4, 4, 4), repulse_threads=256, repulse_xcd_run=0)
+        assert b.tuning == dict(attract_unroll=(4, 4, 4), repulse_threads=256, repulse_xcd_run=32)
 
 
-def test_server_batches_above_the_autotune_size():
+def test_server_large_batch_equals_each_graph_alone():
     reqs = []
     for k, n in enumerate((70_000, 5_000)):  # the first alone is above the size
         n, e = graphs.barabasi_albert(n, 4, seed=60 + k)
