@@ -488,11 +488,11 @@ def main():
                            nodes=n, edges=int(len(e)), precision=args.precision,
                            timed_path="HIP graph replay (two iterations per graph)",
                            launch_shapes=dict(chosen=shapes, candidate_ms=dict(
-                               repulse_threads_64_128_256=shape_ms[:3].tolist(),
-                               repulse_xcd_run_0_8_32_128_512=shape_ms[3:8].tolist(),
-                               attract_lane_unroll_1_2_4_8=shape_ms[8:12].tolist(),
-                               attract_16lane_unroll_1_2_4_8=shape_ms[12:16].tolist(),
-                               attract_wave_unroll_1_2_4_8=shape_ms[16:20].tolist())),
+                               repulse_threads_64_to_1024=shape_ms[:5].tolist(),
+                               repulse_xcd_run_0_8_32_128_512=shape_ms[5:10].tolist(),
+                               attract_lane_unroll_1_2_4_8=shape_ms[10:14].tolist(),
+                               attract_16lane_unroll_1_2_4_8=shape_ms[14:18].tolist(),
+                               attract_wave_unroll_1_2_4_8=shape_ms[18:22].tolist())),
                            stage_ms={k: v * 1e3 for k, v in stages.items()},
                            profiled_ms_per_step=dt_prof / args.steps * 1e3,
                            replay_equals_eager=bool(np.array_equal(pos_replay, pos_out)),

@@ -61,7 +61,7 @@ class Tuning(Structure):
                 ("repulse_xcd_run", c_int)]
 
 
-FA2_TUNE_NTIMES = 20
+FA2_TUNE_NTIMES = 22
 
 
 class FullParams(Structure):

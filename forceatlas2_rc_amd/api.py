@@ -403,7 +403,7 @@ class Batch:
     def tune(self, force=True):
         """Time the candidate launch shapes on the current positions and keep
         the fastest (fa2_batch_tune). Returns the chosen shapes and the
-        timings in ms (traversal at 64, 128, 256 threads; at XCD runs of 0,
+        timings in ms (traversal at 64, 128, 256, 512, 1024 threads; at XCD runs of 0,
         8, 32, 128, 512 blocks; then unrolls 1, 2, 4, 8 of each attraction class),
         or None for timings when a cached choice was reused."""
         t = Tuning()

@@ -875,7 +875,7 @@ __device__ inline void lane_pair_adjust(typename Vec<R>::v4 me, typename Vec<R>:
 // psort holds kLeafSize records of padding after the last node, so that the
 // nodes of a small region are loaded at once.
 template <typename R, bool ANY_ADJUST, bool COUNT = false, bool WIDE = false>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
           const GraphState *__restrict__ gs, const typename Vec<R>::v4 *__restrict__ psort,
           const TNode<R> *__restrict__ tn, const uint32_t *__restrict__ order,
@@ -1477,9 +1477,9 @@ template <typename R> struct EngineImpl final : Engine {
     void drop_graph();
 
     // the repulsion kernel for this batch: with sizes or not, counting or
-    // not, record offsets in 32 or 64 bits; `threads` per workgroup (64,
-    // 128 or 256) and XCD runs of `xrun` blocks (0: none); each wave walks
-    // for the same 64 nodes whatever the shape
+    // not, record offsets in 32 or 64 bits; `threads` per workgroup (64 to
+    // 1024) and XCD runs of `xrun` blocks (0: none); each wave walks for the
+    // same 64 nodes whatever the shape
     template <bool COUNT> void launch_repulse(double2 *Fout, unsigned long long *cnt,
                                               int threads = 0, int xrun = -1)
     {
@@ -2353,8 +2353,10 @@ template <typename R> int EngineImpl<R>::set_tuning(const fa2_tuning *t)
             set_message("attraction unroll %d: 1, 2, 4 or 8", t->attract_unroll[c]);
             return 1;
         }
-    if (t->repulse_threads != 64 && t->repulse_threads != 128 && t->repulse_threads != 256) {
-        set_message("traversal workgroup of %d threads: 64, 128 or 256", t->repulse_threads);
+    if (t->repulse_threads != 64 && t->repulse_threads != 128 && t->repulse_threads != 256 &&
+        t->repulse_threads != 512 && t->repulse_threads != 1024) {
+        set_message("traversal workgroup of %d threads: 64, 128, 256, 512 or 1024",
+                    t->repulse_threads);
         return 1;
     }
     if (t->repulse_xcd_run < 0 || t->repulse_xcd_run > 4096) {
@@ -2422,8 +2424,8 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
     };
     double t[FA2_TUNE_NTIMES] = {0};
     fa2_tuning pick = shape;
-    static const int threads[3] = {64, 128, 256}, unroll[4] = {1, 2, 4, 8};
-    for (int k = 0; k < 3; k++)
+    static const int threads[5] = {64, 128, 256, 512, 1024}, unroll[4] = {1, 2, 4, 8};
+    for (int k = 0; k < 5; k++)
         t[k] = best_of_3([&] { launch_repulse<false>(Fs, nullptr, threads[k], 0); });
     // each default (256 threads, XCD runs of 32 blocks, unroll 4) stays
     // unless another shape is faster by 2%: closer timings are noise
@@ -2434,12 +2436,12 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
             if (tt[k] < tt[b]) b = k;
         return tt[b] < 0.98 * tt[dflt] ? b : dflt;
     };
-    pick.repulse_threads = threads[choose(t, 3, 2)];
+    pick.repulse_threads = threads[choose(t, 5, 2)];
     static const int runs[5] = {0, 8, 32, 128, 512};
     for (int k = 0; k < 5; k++)
-        t[3 + k] = best_of_3(
+        t[5 + k] = best_of_3(
             [&] { launch_repulse<false>(Fs, nullptr, pick.repulse_threads, runs[k]); });
-    pick.repulse_xcd_run = runs[choose(t + 3, 5, 2)];
+    pick.repulse_xcd_run = runs[choose(t + 5, 5, 2)];
     // an attraction kernel runs after the traversal, whose records and
     // nodes fill the caches: each candidate is timed right after one
     for (int c = 0; c < 3; c++) {
@@ -2453,9 +2455,9 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
                     best_of_1([&] { launch_attract_class(c, unroll[k], d_Fa.p, stream); });
                 if (rep == 0 || tk < best) best = tk;
             }
-            t[8 + 4 * c + k] = best;
+            t[10 + 4 * c + k] = best;
         }
-        pick.attract_unroll[c] = unroll[choose(t + 8 + 4 * c, 4, 2)];
+        pick.attract_unroll[c] = unroll[choose(t + 10 + 4 * c, 4, 2)];
     }
     if (e0) (void) hipEventDestroy(e0);
     if (e1) (void) hipEventDestroy(e1);

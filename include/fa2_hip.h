@@ -149,8 +149,8 @@ int fa2_batch_set_state(fa2_batch *b, int g, const double *pos, const double *f_
  * lane has in flight at once (1, 2, 4 or 8) in the attraction kernel of
  * class c (c = 0: nodes of 1..16 incidences, one lane each; 1: 17..256,
  * 16 lanes each; 2: longer lists, one wave per chunk),
- * repulse_threads the workgroup size of the Barnes-Hut traversal (64, 128
- * or 256) and repulse_xcd_run its mapping of blocks to XCDs. The reference has no counterpart (its work sizes are fixed in
+ * repulse_threads the workgroup size of the Barnes-Hut traversal (64, 128,
+ * 256, 512 or 1024) and repulse_xcd_run its mapping of blocks to XCDs. The reference has no counterpart (its work sizes are fixed in
  * src/gpu_full.c:193-198). */
 typedef struct fa2_tuning {
     int attract_unroll[3];
@@ -161,11 +161,11 @@ typedef struct fa2_tuning {
     int repulse_xcd_run;
 } fa2_tuning;
 
-/* Timings behind fa2_batch_tune: the traversal at 64, 128 and 256 threads,
- * then at XCD runs of 0, 8, 32, 128 and 512 blocks, then for each
- * attraction class the unrolls 1, 2, 4 and 8 (ms; 0 where a class is
+/* Timings behind fa2_batch_tune: the traversal at 64, 128, 256, 512 and
+ * 1024 threads, then at XCD runs of 0, 8, 32, 128 and 512 blocks, then for
+ * each attraction class the unrolls 1, 2, 4 and 8 (ms; 0 where a class is
  * empty). */
-#define FA2_TUNE_NTIMES 20
+#define FA2_TUNE_NTIMES 22
 
 /* Autotuner: builds the tree of the batch's current positions and times
  * every candidate launch shape on it, with HIP events on the batch's stream

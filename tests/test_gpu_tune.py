@@ -17,7 +17,7 @@ from forceatlas2_rc_amd import Batch, GPUError, Server, graphs
 pytestmark = pytest.mark.gpu
 
 SHAPES = [((1, 1, 1), 64, 0), ((2, 2, 2), 128, 2), ((8, 8, 8), 256, 32), ((4, 1, 8), 64, 8),
-          ((8, 4, 2), 128, 1), ((4, 4, 4), 256, 4096)]
+          ((8, 4, 2), 128, 1), ((4, 4, 4), 256, 4096), ((2, 4, 8), 512, 8), ((4, 8, 1), 1024, 1)]
 
 
 def _graphs():
@@ -65,8 +65,8 @@ def test_tuning_keeps_the_state_and_a_faster_shape():
     with Batch(gs, autotune=False) as b:
         b.run(3)
         chosen, times = b.tune(force=True)
-        assert times is not None and np.all(times[:3] > 0)
-        assert np.all(times[3:] > 0)  # every class is occupied
+        assert times is not None and np.all(times[:10] > 0)
+        assert np.all(times[10:] > 0)  # every attraction class is occupied
         # the fastest candidate replaces the default (256 threads, XCD runs
         # of 32 blocks, unroll 4) when it was timed at least 2% faster
         def kept(cands, values, chosen_value, default):
@@ -74,10 +74,10 @@ def test_tuning_keeps_the_state_and_a_faster_shape():
             fastest = min(cands, key=lambda c: t[c])
             want = fastest if t[fastest] < 0.98 * t[default] else default
             assert chosen_value == want
-        kept((64, 128, 256), times[:3], chosen["repulse_threads"], 256)
-        kept((0, 8, 32, 128, 512), times[3:8], chosen["repulse_xcd_run"], 32)
+        kept((64, 128, 256, 512, 1024), times[:5], chosen["repulse_threads"], 256)
+        kept((0, 8, 32, 128, 512), times[5:10], chosen["repulse_xcd_run"], 32)
         for c in range(3):
-            kept((1, 2, 4, 8), times[8 + 4 * c:12 + 4 * c], chosen["attract_unroll"][c], 4)
+            kept((1, 2, 4, 8), times[10 + 4 * c:14 + 4 * c], chosen["attract_unroll"][c], 4)
         assert b.tuning == chosen
         b.run(3)
         b.sync()
@@ -93,7 +93,7 @@ def test_tuning_keeps_the_state_and_a_faster_shape():
 def test_tuning_errors():
     gs = _graphs()[1:2]
     with Batch(gs, autotune=False) as b:
-        for bad in (((3, 4, 4), 256, 0), ((4, 4, 4), 96, 0), ((4, 0, 4), 64, 0),
+        for bad in (((3, 4, 4), 256, 0), ((4, 4, 4), 96, 0), ((4, 4, 4), 2048, 0), ((4, 0, 4), 64, 0),
                     ((4, 4, 4), 256, -1), ((4, 4, 4), 256, 5000)):
             with pytest.raises(GPUError):
                 b.set_tuning(*bad)
