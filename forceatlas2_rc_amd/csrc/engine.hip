@@ -137,11 +137,11 @@ k_tile_bbox(const Tile *__restrict__ tiles, const typename Vec<R>::v2 *__restric
 // Bounding box of each graph from its tiles: one wave per graph.
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_graph_bbox(int G, const GraphDesc *__restrict__ gd, const typename Vec<R>::v4 *__restrict__ tbox,
-             typename Vec<R>::v4 *__restrict__ gbox)
+k_graph_bbox(int G, const GraphDesc *__restrict__ gd, const GraphState *__restrict__ gs,
+             const typename Vec<R>::v4 *__restrict__ tbox, typename Vec<R>::v4 *__restrict__ gbox)
 {
     const int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (g >= G) return;
+    if (g >= G || gs[g].retired) return;
     const int lane = lane_id();
     const int t0 = gd[g].toff, nt = gd[g].ntiles;
     R a = (R) INFINITY, b = (R) INFINITY, c = (R) -INFINITY, d = (R) -INFINITY;
@@ -177,14 +177,15 @@ __device__ inline double div_rn(double a, double b) { return __ddiv_rn(a, b); }
 // bit, pipeline.cl:116-130).
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_morton(const Tile *__restrict__ tiles, const typename Vec<R>::v2 *__restrict__ pos,
-         const typename Vec<R>::v4 *__restrict__ gbox, uint32_t *__restrict__ key,
-         uint32_t *__restrict__ val, int UT, int *__restrict__ ucnt)
+k_morton(const Tile *__restrict__ tiles, const GraphState *__restrict__ gs,
+         const typename Vec<R>::v2 *__restrict__ pos, const typename Vec<R>::v4 *__restrict__ gbox,
+         uint32_t *__restrict__ key, uint32_t *__restrict__ val, int UT, int *__restrict__ ucnt)
 {
     // the lengths of the summary unit lists, filled by k_karras
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < UT; q += gridDim.x * blockDim.x)
         ucnt[q * kCountStride] = 0;
     const Tile t = tiles[blockIdx.x];
+    if (gs[t.g].retired) return; // its tree is not used any more
     const typename Vec<R>::v4 bb = gbox[t.g];
     const R ex = bb.z - bb.x, ey = bb.w - bb.y;
     const R extent = ex > ey ? ex : ey;
@@ -214,10 +215,12 @@ constexpr int kTile = kSortThreads * kSortItems; // 2048 nodes
 
 __global__ void __launch_bounds__(kSortThreads)
 k_sort_upsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
-               const uint32_t *__restrict__ key, int shift, uint32_t *__restrict__ hist)
+               const GraphState *__restrict__ gs, const uint32_t *__restrict__ key, int shift,
+               uint32_t *__restrict__ hist)
 {
     __shared__ uint32_t h[256];
     const Tile t = tiles[blockIdx.x];
+    if (gs[t.g].retired) return;
     h[threadIdx.x] = 0;
     __syncthreads();
     for (int i = t.lo + threadIdx.x; i < t.hi; i += kSortThreads)
@@ -229,12 +232,13 @@ k_sort_upsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
 
 // One wave per (graph, digit) row: exclusive scan in place, row total out.
 __global__ void __launch_bounds__(256)
-k_sort_rowscan(int rows, const GraphDesc *__restrict__ gd, uint32_t *__restrict__ hist,
-               uint32_t *__restrict__ rowtot)
+k_sort_rowscan(int rows, const GraphDesc *__restrict__ gd, const GraphState *__restrict__ gs,
+               uint32_t *__restrict__ hist, uint32_t *__restrict__ rowtot)
 {
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= rows) return;
     const int g = r >> 8, d = r & 255;
+    if (gs[g].retired) return;
     const GraphDesc G = gd[g];
     uint32_t *row = hist + (size_t) G.toff * 256 + (size_t) d * G.ntiles;
     const int lane = lane_id();
@@ -256,12 +260,13 @@ k_sort_rowscan(int rows, const GraphDesc *__restrict__ gd, uint32_t *__restrict_
 
 // The same with one block per row, for graphs of more than 64 tiles.
 __global__ void __launch_bounds__(256)
-k_sort_rowscan_block(const GraphDesc *__restrict__ gd, uint32_t *__restrict__ hist,
-                     uint32_t *__restrict__ rowtot)
+k_sort_rowscan_block(const GraphDesc *__restrict__ gd, const GraphState *__restrict__ gs,
+                     uint32_t *__restrict__ hist, uint32_t *__restrict__ rowtot)
 {
     __shared__ uint32_t wsum[4];
     const int r = blockIdx.x;
     const int g = r >> 8, d = r & 255;
+    if (gs[g].retired) return;
     const GraphDesc G = gd[g];
     uint32_t *row = hist + (size_t) G.toff * 256 + (size_t) d * G.ntiles;
     const int lane = lane_id(), w = threadIdx.x >> 6;
@@ -296,7 +301,8 @@ k_sort_rowscan_block(const GraphDesc *__restrict__ gd, uint32_t *__restrict__ hi
 template <typename V4, bool GATHER>
 __global__ void __launch_bounds__(kSortThreads)
 k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
-                 const uint32_t *__restrict__ key_in, const uint32_t *__restrict__ val_in,
+                 const GraphState *__restrict__ gs, const uint32_t *__restrict__ key_in,
+                 const uint32_t *__restrict__ val_in,
                  int shift, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ rowtot,
                  uint32_t *__restrict__ key_out, uint32_t *__restrict__ val_out,
                  const V4 *__restrict__ node4, V4 *__restrict__ psort)
@@ -308,6 +314,7 @@ k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     __shared__ uint32_t scan_tmp[256];
 
     const Tile t = tiles[blockIdx.x];
+    if (gs[t.g].retired) return;
     const GraphDesc G = gd[t.g];
     const int cnt = t.hi - t.lo;
     const int w = threadIdx.x >> 6, lane = lane_id();
@@ -461,7 +468,8 @@ struct Unit {
 // xslot[t] = level << 16 | position in the unit's list, 0 inside a block.
 template <typename R>
 __global__ void k_karras(int NI, const int *__restrict__ gi, const GraphDesc *__restrict__ gd,
-                         const uint32_t *__restrict__ ks, int2 *__restrict__ child,
+                         const GraphState *__restrict__ gs, const uint32_t *__restrict__ ks,
+                         int2 *__restrict__ child,
                          TNode<R> *__restrict__ tn, int *__restrict__ parent_leaf,
                          int *__restrict__ parent_int, int *__restrict__ escape,
                          const int *__restrict__ ubase, int *__restrict__ ucnt,
@@ -470,6 +478,7 @@ __global__ void k_karras(int NI, const int *__restrict__ gi, const GraphDesc *__
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= NI) return;
     const int g = gi[t];
+    if (gs[g].retired) return;
     const GraphDesc G = gd[g];
     const int n = G.n, i = t - G.ioff, base = G.noff;
     const uint32_t *k = ks + base;
@@ -579,6 +588,7 @@ constexpr int kOutside = 1 << 24;
 template <typename R>
 __global__ void __launch_bounds__(1024)
 k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
+                 const GraphState *__restrict__ gs,
                  const typename Vec<R>::v4 *__restrict__ psort, const int2 *__restrict__ child,
                  const int *__restrict__ parent_leaf, const int *__restrict__ parent_int,
                  const int *__restrict__ escape, int NI, typename Vec<R>::v4 *nd, TNode<R> *tn)
@@ -595,7 +605,7 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     const int g = T.g;
     const int lo = T.lo + (int) (blockIdx.x % kParts) * B;
     const int hi = min(T.hi, lo + B);
-    if (lo >= hi) return;
+    if (lo >= hi || gs[g].retired) return;
     const GraphDesc D = gd[g];
     const int gend = D.noff + D.n;
     const int tbase = lo - g; // the regions that may lie inside: tbase + [0, hi - lo)
@@ -711,6 +721,7 @@ k_aggregate_unit(const Unit *__restrict__ units, int level, const GraphDesc *__r
     __shared__ int s_skip[kUnitCap];
     __shared__ R s_sum[7][kUnitCap]; // x0, y0, x1, y1, m, mx, my
     const Unit U = units[blockIdx.x];
+    if (gs[U.g].retired) return;
     const GraphDesc D = gd[U.g];
     const int gend = D.noff + D.n;
     const int cnt = ucnt[U.index * kCountStride];
@@ -1876,38 +1887,39 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
 {
     mark(ev);
     // 1-2: codes
-    k_graph_bbox<R><<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_tbox.p, d_gbox.p);
-    k_morton<R><<<T, 256, 0, stream>>>(d_tiles.p, d_pos.p, d_gbox.p, d_keyA.p, d_valA.p,
+    k_graph_bbox<R><<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_gs.p, d_tbox.p, d_gbox.p);
+    k_morton<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gs.p, d_pos.p, d_gbox.p, d_keyA.p, d_valA.p,
                                        (int) h_units.size(), d_ucnt.p);
     mark(ev);
     // 3: sort, A -> B -> A -> B -> A; the last pass gathers psort
     for (int pass = 0; pass < 4; pass++) {
         uint32_t *ki = pass % 2 ? d_keyB.p : d_keyA.p, *vi = pass % 2 ? d_valB.p : d_valA.p;
         uint32_t *ko = pass % 2 ? d_keyA.p : d_keyB.p, *vo = pass % 2 ? d_valA.p : d_valB.p;
-        k_sort_upsweep<<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, ki, 8 * pass, d_hist.p);
+        k_sort_upsweep<<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, ki, 8 * pass,
+                                                        d_hist.p);
         if (max_tiles > 64)
-            k_sort_rowscan_block<<<256 * G, 256, 0, stream>>>(d_gd.p, d_hist.p, d_rowtot.p);
+            k_sort_rowscan_block<<<256 * G, 256, 0, stream>>>(d_gd.p, d_gs.p, d_hist.p, d_rowtot.p);
         else
-            k_sort_rowscan<<<ceil_div(256 * (size_t) G, 4), 256, 0, stream>>>(256 * G, d_gd.p,
-                                                                              d_hist.p, d_rowtot.p);
+            k_sort_rowscan<<<ceil_div(256 * (size_t) G, 4), 256, 0, stream>>>(
+                256 * G, d_gd.p, d_gs.p, d_hist.p, d_rowtot.p);
         if (pass < 3)
             k_sort_downsweep<V4, false><<<T, kSortThreads, 0, stream>>>(
-                d_tiles.p, d_gd.p, ki, vi, 8 * pass, d_hist.p, d_rowtot.p, ko, vo, nullptr,
+                d_tiles.p, d_gd.p, d_gs.p, ki, vi, 8 * pass, d_hist.p, d_rowtot.p, ko, vo, nullptr,
                 nullptr);
         else
             k_sort_downsweep<V4, true><<<T, kSortThreads, 0, stream>>>(
-                d_tiles.p, d_gd.p, ki, vi, 8 * pass, d_hist.p, d_rowtot.p, ko, vo, d_node4.p,
+                d_tiles.p, d_gd.p, d_gs.p, ki, vi, 8 * pass, d_hist.p, d_rowtot.p, ko, vo, d_node4.p,
                 d_psort.p);
     }
     mark(ev);
     // 4: tree
     if (NI > 0) {
         k_karras<R><<<ceil_div(NI, 256), 256, 0, stream>>>(
-            NI, d_gi.p, d_gd.p, d_keyA.p, d_child.p, d_tn.p, d_parent_leaf.p, d_parent_int.p,
+            NI, d_gi.p, d_gd.p, d_gs.p, d_keyA.p, d_child.p, d_tn.p, d_parent_leaf.p, d_parent_int.p,
             d_escape.p, d_ubase.p, d_ucnt.p, d_lists.p, d_xslot.p);
         k_aggregate_tile<R><<<T * (kTile / agg_block<R>()), 1024, 0, stream>>>(
-            d_tiles.p, d_gd.p, d_psort.p, d_child.p, d_parent_leaf.p, d_parent_int.p, d_escape.p,
-            NI, d_nd.p, d_tn.p);
+            d_tiles.p, d_gd.p, d_gs.p, d_psort.p, d_child.p, d_parent_leaf.p, d_parent_int.p,
+            d_escape.p, NI, d_nd.p, d_tn.p);
         for (int L = 2; L < kMaxLevels; L++)
             if (level_cnt[L] > 0)
                 k_aggregate_unit<R><<<level_cnt[L], 1024, 0, stream>>>(
