@@ -108,16 +108,17 @@ def dfs_skip(child, n):
 
 
 @pytest.mark.parametrize("precision,n,mode", [("single", 3000, "f"), ("single", 9000, "f"),
-                                              ("single", 2048, "f"), ("double", 3000, "d"),
-                                              ("double", 1500, "d"), ("double", 1024, "d")])
+                                              ("single", 2048, "f"), ("single", 70000, "f"),
+                                              ("double", 3000, "d"), ("double", 1500, "d"),
+                                              ("double", 1024, "d"), ("double", 70000, "d")])
 def test_region_records(precision, n, mode):
     """Every region's traversal record (pipeline.cl:276-323) against the
     oracle on the same positions: masses and centres bit-exact, size^2
     within one rounding (the device may fuse w * w + h * h); skip links are
-    the depth-first successors in the oracle's tree. 9000 nodes span five
-    tiles, so regions are summarised at every level; in double precision
-    the first level works on half tiles, so 1500 nodes already need the
-    second."""
+    the depth-first successors in the oracle's tree. The blocks of the first
+    summary level hold 1024 nodes in single precision and 512 in double, and
+    the units above them 16 and 256 blocks: 1024-9000 nodes need the second
+    level (1024 in double is two blocks), 70000 nodes the third."""
     n, e = pa(n)
     pos = start(n)
     if precision == "single":
