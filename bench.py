@@ -9,17 +9,20 @@ iteration of the whole graph: codes, sort, tree, repulsion, attraction,
 speed, displacement. The graph and its state are resident in HBM before the
 timed region starts.
 
-The timed iterations are replayed from the captured HIP graph, as a layout
-runs. The same number of iterations is then launched one kernel at a time
-with HIP events on the batch's stream between the phases: their times give
-the phase breakdown, the roofline of the dominant kernel (the Barnes-Hut
-traversal, FP32 VALU) and the iteration roofline (per phase, the larger of
-minimum HBM bytes over 8 TB/s and FLOP over the FP32 peak, summed, over the
-measured iteration time), with each phase's HBM rate.
+Before the timed region the autotuner times its candidate launch shapes
+on the graph and keeps the fastest. The timed iterations are replayed from
+the captured HIP graph, as a layout runs. The same number of iterations is
+then launched one kernel at a time with HIP events on the batch's stream
+between the phases: their times give the phase breakdown, the roofline of
+the dominant kernel (the Barnes-Hut traversal, FP32 VALU) and the iteration
+roofline (per phase, the larger of minimum HBM bytes over 8 TB/s and FLOP
+over the FP32 peak, summed, over the measured iteration time), with each
+phase's HBM rate.
 
-Printed: one JSON line with node-iters/s, the rooflines and the CPU baseline (the C restatement of the reference CPU layout,
-Gephi Barnes-Hut in FP64 with OpenMP, on a bounded sample of iterations of
-the same graph). Beside it, in the same line:
+Printed: one JSON line with node-iters/s, the rooflines and the CPU
+baseline (the C restatement of the reference CPU layout, Gephi Barnes-Hut
+in FP64 with OpenMP, on a bounded sample of iterations of the same graph).
+Beside it, in the same line:
   * "allpairs": the exact all-pairs repulsion kernel of the offload path on
     a 50,000-node preferential-attachment graph, FP32, its FLOP/s against
     the FP32 peak (12 FLOP per ordered pair);
