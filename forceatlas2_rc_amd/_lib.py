@@ -58,10 +58,10 @@ class Edges(Structure):
 class Tuning(Structure):
     """fa2_tuning (include/fa2_hip.h): launch shapes of a batch."""
     _fields_ = [("attract_unroll", c_int * 3), ("repulse_threads", c_int),
-                ("repulse_xcd_run", c_int)]
+                ("repulse_xcd_run", c_int), ("attract_fork", c_int)]
 
 
-FA2_TUNE_NTIMES = 22
+FA2_TUNE_NTIMES = 26
 
 
 class FullParams(Structure):

@@ -346,7 +346,8 @@ def _fill_params(p, gr, device, precision, profile, edges=None):
 
 def _tuning_dict(t):
     return dict(attract_unroll=tuple(int(t.attract_unroll[c]) for c in range(3)),
-                repulse_threads=int(t.repulse_threads), repulse_xcd_run=int(t.repulse_xcd_run))
+                repulse_threads=int(t.repulse_threads), repulse_xcd_run=int(t.repulse_xcd_run),
+                attract_fork=int(t.attract_fork))
 
 
 class Batch:
@@ -404,7 +405,8 @@ class Batch:
         """Time the candidate launch shapes on the current positions and keep
         the fastest (fa2_batch_tune). Returns the chosen shapes and the
         timings in ms (traversal at 64, 128, 256, 512, 1024 threads; at XCD runs of 0,
-        8, 32, 128, 512 blocks; then unrolls 1, 2, 4, 8 of each attraction class),
+        8, 32, 128, 512 blocks; unrolls 1, 2, 4, 8 of each attraction class; then
+        phases 1-6 of an iteration with the attraction forked at 0, 1, 2, 3),
         or None for timings when a cached choice was reused."""
         t = Tuning()
         times = np.full(FA2_TUNE_NTIMES, -1.0)
@@ -419,12 +421,14 @@ class Batch:
             raise GPUError(last_error())
         return _tuning_dict(t)
 
-    def set_tuning(self, attract_unroll=(4, 4, 4), repulse_threads=256, repulse_xcd_run=32):
+    def set_tuning(self, attract_unroll=(4, 4, 4), repulse_threads=256, repulse_xcd_run=32,
+                   attract_fork=2):
         t = Tuning()
         for c in range(3):
             t.attract_unroll[c] = int(attract_unroll[c])
         t.repulse_threads = int(repulse_threads)
         t.repulse_xcd_run = int(repulse_xcd_run)
+        t.attract_fork = int(attract_fork)
         if lib.fa2_batch_set_tuning(self.h, ctypes.byref(t)) != 0:
             raise GPUError(last_error())
 

@@ -1439,7 +1439,7 @@ template <typename R> struct EngineImpl final : Engine {
     hipGraphExec_t graph_exec = nullptr;
 
     // launch shapes (none changes a result)
-    fa2_tuning shape = {{4, 4, 4}, 256, 32};
+    fa2_tuning shape = {{4, 4, 4}, 256, 32, 2};
 
     ~EngineImpl() override
     {
@@ -1479,8 +1479,15 @@ template <typename R> struct EngineImpl final : Engine {
     void enqueue_iteration(std::vector<hipEvent_t> *ev);
     void mark(std::vector<hipEvent_t> *ev);
     // phases 1-4: codes, sort, tree of the current positions (scratch
-    // buffers only)
-    void enqueue_tree(std::vector<hipEvent_t> *ev);
+    // buffers only); `fork` (0, 1, 2) starts the attraction on the second
+    // stream before the codes, after the sort or after the tree, -1 never
+    void enqueue_tree(std::vector<hipEvent_t> *ev, int fork = -1);
+    // phases 1-6 without profiling events: the attraction forked at `fork`
+    // (0..2) on the second stream and joined after the traversal, or (3)
+    // launched after the traversal on the batch's stream; the traversal
+    // writes Fout
+    void enqueue_forces(double2 *Fout, int fork);
+    void fork_attraction();
     // the attraction kernel of class c at unroll u, writing into Fa
     void launch_attract_class(int c, int u, double2 *Fa, hipStream_t st);
     // every attraction kernel of an iteration, on st
@@ -1850,27 +1857,42 @@ template <typename R> void EngineImpl<R>::launch_attraction(hipStream_t st)
             (int) h_hubs.size(), d_hubs.p, d_gid.p, d_gs.p, d_hubpart.p, d_Fa.p);
 }
 
+template <typename R> void EngineImpl<R>::fork_attraction()
+{
+    (void) hipEventRecord(ev_fork, stream);
+    (void) hipStreamWaitEvent(s_att, ev_fork, 0);
+    launch_attraction(s_att);
+    (void) hipEventRecord(ev_join, s_att);
+}
+
+template <typename R> void EngineImpl<R>::enqueue_forces(double2 *Fout, int fork)
+{
+    if (!s_att) fork = 3;
+    enqueue_tree(nullptr, fork < 3 ? fork : -1);
+    // 5: repulsion minus gravity
+    launch_repulse<false>(Fout, nullptr);
+    if (fork < 3) (void) hipStreamWaitEvent(stream, ev_join, 0);
+    else launch_attraction(stream);
+}
+
 template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent_t> *ev)
 {
     double2 *Fc = d_F[parity].p, *Fo = d_F[1 - parity].p;
     // 6: the attraction needs only the positions of the last displacement.
-    // Without profiling it runs on the second stream, forked here (after the
-    // last displacement) and joined before the swing, beside phases 1-5,
-    // which are bound by other units (scalar loads and VALU, not the
-    // vector memory pipeline of its gathers).
-    const bool side = !ev && s_att && overlap;
-    if (side) {
-        (void) hipEventRecord(ev_fork, stream);
-        (void) hipStreamWaitEvent(s_att, ev_fork, 0);
-        launch_attraction(s_att);
-        (void) hipEventRecord(ev_join, s_att);
+    // Without profiling it runs on the second stream, forked at the point
+    // the launch shapes name and joined before the swing. Beside the
+    // traversal (the default) it overlaps best: the traversal is bound by
+    // its VALU work, the attraction by the gathers of the vector memory
+    // pipeline; beside the codes and the sort, which wait on memory too,
+    // both slow down.
+    if (!ev && overlap) {
+        enqueue_forces(Fc, shape.attract_fork);
+    } else {
+        enqueue_tree(ev);
+        launch_repulse<false>(Fc, nullptr);
+        mark(ev);
+        launch_attraction(stream);
     }
-    enqueue_tree(ev);
-    // 5: repulsion minus gravity
-    launch_repulse<false>(Fc, nullptr);
-    mark(ev);
-    if (side) (void) hipStreamWaitEvent(stream, ev_join, 0);
-    else launch_attraction(stream);
     mark(ev);
     // 7-8: the force (k_swing adds the attraction), swinging, speed,
     // displacement
@@ -1883,8 +1905,9 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
     parity = 1 - parity;
 }
 
-template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *ev)
+template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *ev, int fork)
 {
+    if (fork == 0) fork_attraction();
     mark(ev);
     // 1-2: codes
     k_graph_bbox<R><<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_gs.p, d_tbox.p, d_gbox.p);
@@ -1912,6 +1935,7 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
                 d_psort.p);
     }
     mark(ev);
+    if (fork == 1) fork_attraction();
     // 4: tree
     if (NI > 0) {
         k_karras<R><<<ceil_div(NI, 256), 256, 0, stream>>>(
@@ -1928,6 +1952,7 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
                     d_tn.p);
     }
     mark(ev);
+    if (fork == 2) fork_attraction();
 }
 
 // Record two iterations into a HIP graph (no memory set or copy is part of
@@ -2375,6 +2400,10 @@ template <typename R> int EngineImpl<R>::set_tuning(const fa2_tuning *t)
         set_message("XCD run of %d blocks: 0 to 4096", t->repulse_xcd_run);
         return 1;
     }
+    if (t->attract_fork < 0 || t->attract_fork > 3) {
+        set_message("attraction fork %d: 0, 1, 2 or 3", t->attract_fork);
+        return 1;
+    }
     (void) hipSetDevice(device);
     (void) hipStreamSynchronize(stream); // a captured graph may still be running
     shape = *t;
@@ -2470,6 +2499,15 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
             t[10 + 4 * c + k] = best;
         }
         pick.attract_unroll[c] = unroll[choose(t + 10 + 4 * c, 4, 2)];
+    }
+    // where the attraction runs beside the other phases: phases 1-6 with the
+    // shapes chosen so far (without a second stream only 3 is possible)
+    if (s_att && overlap) {
+        const fa2_tuning keep = shape;
+        shape = pick;
+        for (int f = 0; f < 4; f++) t[22 + f] = best_of_3([&] { enqueue_forces(Fs, f); });
+        shape = keep;
+        pick.attract_fork = choose(t + 22, 4, 2);
     }
     if (e0) (void) hipEventDestroy(e0);
     if (e1) (void) hipEventDestroy(e1);

@@ -150,8 +150,10 @@ int fa2_batch_set_state(fa2_batch *b, int g, const double *pos, const double *f_
  * class c (c = 0: nodes of 1..16 incidences, one lane each; 1: 17..256,
  * 16 lanes each; 2: longer lists, one wave per chunk),
  * repulse_threads the workgroup size of the Barnes-Hut traversal (64, 128,
- * 256, 512 or 1024) and repulse_xcd_run its mapping of blocks to XCDs. The reference has no counterpart (its work sizes are fixed in
- * src/gpu_full.c:193-198). */
+ * 256, 512 or 1024), repulse_xcd_run its mapping of blocks to XCDs and
+ * attract_fork where the attraction runs beside the other phases. The
+ * reference has no counterpart (its work sizes are fixed in
+ * src/gpu_full.c:193-198, its kernels run one after the other). */
 typedef struct fa2_tuning {
     int attract_unroll[3];
     int repulse_threads;
@@ -159,20 +161,27 @@ typedef struct fa2_tuning {
      * many consecutive blocks (0..4096; default 32), so that one L2 serves
      * the tree records of neighbouring nodes */
     int repulse_xcd_run;
+    /* where the attraction kernels start on the batch's second stream:
+     * 0 with the iteration (beside the codes, sort and tree), 1 after the
+     * sort (beside the tree), 2 after the tree (beside the traversal; the
+     * default), 3 after the traversal on the batch's own stream. Iterations
+     * launched with profiling events, and server batches, always use 3. */
+    int attract_fork;
 } fa2_tuning;
 
 /* Timings behind fa2_batch_tune: the traversal at 64, 128, 256, 512 and
  * 1024 threads, then at XCD runs of 0, 8, 32, 128 and 512 blocks, then for
  * each attraction class the unrolls 1, 2, 4 and 8 (ms; 0 where a class is
- * empty). */
-#define FA2_TUNE_NTIMES 22
+ * empty), then phases 1-6 of an iteration (codes, sort, tree, traversal and
+ * attraction) with the attraction forked at 0, 1, 2 and 3. */
+#define FA2_TUNE_NTIMES 26
 
 /* Autotuner: builds the tree of the batch's current positions and times
  * every candidate launch shape on it, with HIP events on the batch's stream
  * (best of three launches each; an attraction candidate right after a
  * traversal launch, as in an iteration), then keeps for the following
- * iterations each default shape (256 threads, XCD runs of 32, unroll 4) unless
- * another was timed at least 2% faster. The layout state is left as it was: the candidates write only
+ * iterations each default shape (256 threads, XCD runs of 32, unroll 4, fork
+ * 2) unless another was timed at least 2% faster. The layout state is left as it was: the candidates write only
  * into the force buffer that the next iteration overwrites. Choices are
  * cached per process by device, precision and the batch's size class; with
  * force == 0 a cached choice is reused without timing (times then stay

@@ -57,6 +57,23 @@ def test_struct_layouts_match_the_c_header(tmp_path):
     assert got == want
 
 
+def test_tuning_layout_matches_the_c_header(tmp_path):
+    """The ctypes mirror of fa2_tuning has the layout of include/fa2_hip.h."""
+    src = tmp_path / "tun.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "fa2_hip.h"\n'
+        "int main(void){printf(\"%zu %zu %zu %d\\n\", sizeof(fa2_tuning),"
+        "offsetof(fa2_tuning, repulse_xcd_run), offsetof(fa2_tuning, attract_fork),"
+        "FA2_TUNE_NTIMES);return 0;}\n")
+    exe = tmp_path / "tun"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True,
+                                          capture_output=True, text=True).stdout.split()]
+    assert got == [ctypes.sizeof(_lib.Tuning), _lib.Tuning.repulse_xcd_run.offset,
+                   _lib.Tuning.attract_fork.offset, _lib.FA2_TUNE_NTIMES]
+
+
 def test_clock_is_monotonic():
     a = _lib.lib.fa2_now()
     b = _lib.lib.fa2_now()

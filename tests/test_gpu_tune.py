@@ -1,8 +1,9 @@
 """The autotuner of a batch's launch shapes (fa2_batch_tune,
 fa2_batch_set_tuning of include/fa2_hip.h).
 
-Gates: every candidate shape gives positions bit-identical to the default
-shapes, with the iterations replayed from the captured HIP graph (which is
+Gates: every candidate shape (the attraction's fork point included) gives
+positions bit-identical to the default shapes, with the iterations replayed
+from the captured HIP graph (which is
 recaptured after each change) and launched kernel by kernel; tuning leaves
 the layout state untouched; the tuner keeps a shape other than the default
 only when it timed it at least 2% faster;
@@ -16,8 +17,9 @@ from forceatlas2_rc_amd import Batch, GPUError, Server, graphs
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [((1, 1, 1), 64, 0), ((2, 2, 2), 128, 2), ((8, 8, 8), 256, 32), ((4, 1, 8), 64, 8),
-          ((8, 4, 2), 128, 1), ((4, 4, 4), 256, 4096), ((2, 4, 8), 512, 8), ((4, 8, 1), 1024, 1)]
+SHAPES = [((1, 1, 1), 64, 0, 0), ((2, 2, 2), 128, 2, 1), ((8, 8, 8), 256, 32, 3),
+          ((4, 1, 8), 64, 8, 2), ((8, 4, 2), 128, 1, 0), ((4, 4, 4), 256, 4096, 1),
+          ((2, 4, 8), 512, 8, 3), ((4, 8, 1), 1024, 1, 2)]
 
 
 def _graphs():
@@ -66,9 +68,11 @@ def test_tuning_keeps_the_state_and_a_faster_shape():
         b.run(3)
         chosen, times = b.tune(force=True)
         assert times is not None and np.all(times[:10] > 0)
-        assert np.all(times[10:] > 0)  # every attraction class is occupied
+        assert np.all(times[10:22] > 0)  # every attraction class is occupied
+        assert np.all(times[22:26] > 0)
         # the fastest candidate replaces the default (256 threads, XCD runs
-        # of 32 blocks, unroll 4) when it was timed at least 2% faster
+        # of 32 blocks, unroll 4, attraction forked after the tree) when it
+        # was timed at least 2% faster
         def kept(cands, values, chosen_value, default):
             t = dict(zip(cands, values))
             fastest = min(cands, key=lambda c: t[c])
@@ -78,6 +82,7 @@ def test_tuning_keeps_the_state_and_a_faster_shape():
         kept((0, 8, 32, 128, 512), times[5:10], chosen["repulse_xcd_run"], 32)
         for c in range(3):
             kept((1, 2, 4, 8), times[10 + 4 * c:14 + 4 * c], chosen["attract_unroll"][c], 4)
+        kept((0, 1, 2, 3), times[22:26], chosen["attract_fork"], 2)
         assert b.tuning == chosen
         b.run(3)
         b.sync()
@@ -94,10 +99,12 @@ def test_tuning_errors():
     gs = _graphs()[1:2]
     with Batch(gs, autotune=False) as b:
         for bad in (((3, 4, 4), 256, 0), ((4, 4, 4), 96, 0), ((4, 4, 4), 2048, 0), ((4, 0, 4), 64, 0),
-                    ((4, 4, 4), 256, -1), ((4, 4, 4), 256, 5000)):
+                    ((4, 4, 4), 256, -1), ((4, 4, 4), 256, 5000), ((4, 4, 4), 256, 32, -1),
+                    ((4, 4, 4), 256, 32, 4)):
             with pytest.raises(GPUError):
                 b.set_tuning(*bad)
-        assert b.tuning == dict(attract_unroll=(4, 4, 4), repulse_threads=256, repulse_xcd_run=32)
+        assert b.tuning == dict(attract_unroll=(4, 4, 4), repulse_threads=256, repulse_xcd_run=32,
+                                attract_fork=2)
 
 
 def test_server_large_batch_equals_each_graph_alone():
