@@ -422,13 +422,10 @@ __global__ void k_pack_node4(int N, const typename Vec<R>::v2 *__restrict__ pos,
     node4[i] = mk4<R>(p.x, p.y, mass[i], size[i]);
 }
 
-// Common prefix of sorted keys a and b of one graph (local positions),
-// equal keys told apart by their positions; -1 out of range
-// (pipeline.cl:211-217).
-__device__ inline int kdelta(const uint32_t *__restrict__ ks, int n, int a, int b)
+// Common prefix of the sorted keys x (at local position a) and y (at b) of
+// one graph, equal keys told apart by their positions (pipeline.cl:211-217).
+__device__ inline int kdelta(uint32_t x, uint32_t y, int a, int b)
 {
-    if (b < 0 || b >= n) return -1;
-    const uint32_t x = ks[a], y = ks[b];
     if (x == y) return 32 + __clz((int) ((uint32_t) a ^ (uint32_t) b));
     return __clz((int) (x ^ y));
 }
@@ -466,35 +463,62 @@ struct Unit {
 // node a traversal visits next (the right child of the region split there),
 // and lists the region in its summary unit when that is of level >= 2:
 // xslot[t] = level << 16 | position in the unit's list, 0 inside a block.
+//
+// The searches for a region's range and split compare its key with keys at
+// growing distances, one dependent load after another. The workgroup of
+// kKarrasThreads regions first stages the sorted keys of its positions and
+// kKarrasWin more on each side in LDS (region t of graph g sits at global
+// position t + g, so a workgroup's positions span at most 2 * kKarrasThreads
+// - 1, with graphs of two nodes); only the probes of the few regions whose
+// range reaches beyond the window load from global memory.
+constexpr int kKarrasThreads = 256;
+constexpr int kKarrasWin = 1024;
+constexpr int kKarrasStage = 2 * kKarrasThreads + 2 * kKarrasWin;
+
 template <typename R>
-__global__ void k_karras(int NI, const int *__restrict__ gi, const GraphDesc *__restrict__ gd,
-                         const GraphState *__restrict__ gs, const uint32_t *__restrict__ ks,
-                         int2 *__restrict__ child,
-                         TNode<R> *__restrict__ tn, int *__restrict__ parent_leaf,
-                         int *__restrict__ parent_int, int *__restrict__ escape,
-                         const int *__restrict__ ubase, int *__restrict__ ucnt,
-                         int *__restrict__ lists, int *__restrict__ xslot)
+__global__ void __launch_bounds__(kKarrasThreads)
+k_karras(int NI, int N, const int *__restrict__ gi, const GraphDesc *__restrict__ gd,
+         const GraphState *__restrict__ gs, const uint32_t *__restrict__ ks,
+         int2 *__restrict__ child, TNode<R> *__restrict__ tn, int *__restrict__ parent_leaf,
+         int *__restrict__ parent_int, int *__restrict__ escape, const int *__restrict__ ubase,
+         int *__restrict__ ucnt, int *__restrict__ lists, int *__restrict__ xslot)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t s_key[kKarrasStage];
+    const int t0 = (int) blockIdx.x * kKarrasThreads;
+    const int t = t0 + (int) threadIdx.x;
+    const int tl = min(t0 + kKarrasThreads - 1, NI - 1);
+    const int lo = t0 + gi[t0] - kKarrasWin; // s_key[q] holds the key at position lo + q
+    const int cnt = min(tl + gi[tl] + kKarrasWin + 1 - lo, kKarrasStage);
+    for (int q = (int) threadIdx.x; q < cnt; q += kKarrasThreads) {
+        const int p = lo + q;
+        s_key[q] = p >= 0 && p < N ? ks[p] : 0u;
+    }
+    __syncthreads();
     if (t >= NI) return;
     const int g = gi[t];
     if (gs[g].retired) return;
     const GraphDesc G = gd[g];
     const int n = G.n, i = t - G.ioff, base = G.noff;
-    const uint32_t *k = ks + base;
-    const int d = kdelta(k, n, i, i + 1) > kdelta(k, n, i, i - 1) ? 1 : -1;
-    const int dmin = kdelta(k, n, i, i - d);
+    auto key = [&](int b) -> uint32_t { // key at local position b of the graph
+        const int q = base + b - lo;
+        return (unsigned) q < (unsigned) cnt ? s_key[q] : ks[base + b];
+    };
+    const uint32_t ki = key(i);
+    // δ(i, b), -1 out of the graph
+    auto delta = [&](int b) -> int { return b < 0 || b >= n ? -1 : kdelta(ki, key(b), i, b); };
+    const int d = delta(i + 1) > delta(i - 1) ? 1 : -1;
+    const int dmin = delta(i - d);
     int lmax = 2;
-    while (kdelta(k, n, i, i + lmax * d) > dmin) lmax *= 2;
+    while (delta(i + lmax * d) > dmin) lmax *= 2;
     int l = 0;
     for (int s = lmax / 2; s >= 1; s /= 2)
-        if (kdelta(k, n, i, i + (l + s) * d) > dmin) l += s;
+        if (delta(i + (l + s) * d) > dmin) l += s;
     const int j = i + l * d;
-    const int dnode = kdelta(k, n, i, j);
+    const int dnode = delta(j);
     int s = 0, st = l;
     do {
         st = (st + 1) >> 1;
-        if (kdelta(k, n, i, i + (s + st) * d) > dnode) s += st;
+        if (delta(i + (s + st) * d) > dnode) s += st;
     } while (st > 1);
     const int split = i + s * d + min(d, 0);
     const int first = min(i, j), last = max(i, j);
@@ -1938,8 +1962,8 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
     if (fork == 1) fork_attraction();
     // 4: tree
     if (NI > 0) {
-        k_karras<R><<<ceil_div(NI, 256), 256, 0, stream>>>(
-            NI, d_gi.p, d_gd.p, d_gs.p, d_keyA.p, d_child.p, d_tn.p, d_parent_leaf.p, d_parent_int.p,
+        k_karras<R><<<ceil_div(NI, kKarrasThreads), kKarrasThreads, 0, stream>>>(
+            NI, N, d_gi.p, d_gd.p, d_gs.p, d_keyA.p, d_child.p, d_tn.p, d_parent_leaf.p, d_parent_int.p,
             d_escape.p, d_ubase.p, d_ucnt.p, d_lists.p, d_xslot.p);
         k_aggregate_tile<R><<<T * (kTile / agg_block<R>()), 1024, 0, stream>>>(
             d_tiles.p, d_gd.p, d_gs.p, d_psort.p, d_child.p, d_parent_leaf.p, d_parent_int.p,
