@@ -222,9 +222,19 @@ k_sort_upsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
     const Tile t = tiles[blockIdx.x];
     if (gs[t.g].retired) return;
     h[threadIdx.x] = 0;
+    // the thread's keys, every load in flight at once (a slot past the
+    // tile's end reads the tile's last key and is not counted)
+    uint32_t k8[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++)
+        k8[r] = key[min(t.lo + (int) threadIdx.x + r * kSortThreads, t.hi - 1)];
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) asm volatile("" : "+v"(k8[r]));
     __syncthreads();
-    for (int i = t.lo + threadIdx.x; i < t.hi; i += kSortThreads)
-        atomicAdd(&h[(key[i] >> shift) & 255u], 1u);
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++)
+        if (t.lo + (int) threadIdx.x + r * kSortThreads < t.hi)
+            atomicAdd(&h[(k8[r] >> shift) & 255u], 1u);
     __syncthreads();
     const GraphDesc g = gd[t.g];
     hist[(size_t) g.toff * 256 + (size_t) threadIdx.x * g.ntiles + t.j] = h[threadIdx.x];
@@ -322,15 +332,27 @@ k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     for (int k = 0; k < 4; k++) wcnt[k][tid] = 0;
     __syncthreads();
 
+    // the wave's keys and values, and the tile's row offsets and the
+    // graph's digit totals used further down: every load in flight at once
+    // (a slot past the tile's end reads its last key and takes no part)
     uint32_t kk[kSortItems], vv[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const int q = min(w * (kSortItems * 64) + r * 64 + lane, cnt - 1);
+        kk[r] = key_in[t.lo + q];
+        vv[r] = val_in[t.lo + q];
+    }
+    uint32_t rt = rowtot[t.g * 256 + tid];
+    uint32_t pre = hist[(size_t) G.toff * 256 + (size_t) tid * G.ntiles + t.j];
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) asm volatile("" : "+v"(kk[r]), "+v"(vv[r]));
+    asm volatile("" : "+v"(rt), "+v"(pre));
     int dig[kSortItems], rank[kSortItems];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
         const int q = w * (kSortItems * 64) + r * 64 + lane;
         const bool ok = q < cnt;
-        kk[r] = ok ? key_in[t.lo + q] : 0u;
-        vv[r] = ok ? val_in[t.lo + q] : 0u;
         const int d = (int) ((kk[r] >> shift) & 255u);
         dig[r] = ok ? d : -1;
         uint64_t peers = __ballot(ok);
@@ -356,7 +378,6 @@ k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     // each wave by shuffles, then across the four waves.
     const uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
     const uint32_t tot = c0 + c1 + c2 + c3;
-    const uint32_t rt = rowtot[t.g * 256 + tid];
     uint32_t a = tot, b = rt; // inclusive scans within the wave
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -382,7 +403,6 @@ k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     wcnt[1][tid] = start + c0;
     wcnt[2][tid] = start + c0 + c1;
     wcnt[3][tid] = start + c0 + c1 + c2;
-    const uint32_t pre = hist[(size_t) G.toff * 256 + (size_t) tid * G.ntiles + t.j];
     goff[tid] = (int) (G.noff + gbase + pre) - (int) start;
     __syncthreads();
 
@@ -394,15 +414,37 @@ k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
         sval[p] = vv[r];
     }
     __syncthreads();
-    for (int p = tid; p < cnt; p += kSortThreads) {
-        const uint32_t k = skey[p];
-        const int d = (int) ((k >> shift) & 255u);
-        const int dst = goff[d] + p;
-        if (dst < G.noff || dst >= G.noff + G.n) continue; // never taken
-        const uint32_t v = sval[p];
-        key_out[dst] = k;
-        val_out[dst] = v;
-        if (GATHER) psort[dst] = node4[v];
+    // each thread moves kSortItems keys; with GATHER, the (x, y, mass,
+    // radius) loads of all of them are issued before the first store, so
+    // that their latencies overlap (a load past the tile's end reads node 0
+    // and is not stored)
+    uint32_t ko[kSortItems], vo[kSortItems];
+    int dsto[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const int p = tid + r * kSortThreads;
+        const bool in = p < cnt;
+        ko[r] = in ? skey[p] : 0u;
+        vo[r] = in ? sval[p] : 0u;
+        dsto[r] = in ? goff[(ko[r] >> shift) & 255u] + p : -1;
+    }
+    V4 nv[GATHER ? kSortItems : 1];
+    if (GATHER) {
+#pragma unroll
+        for (int r = 0; r < kSortItems; r++) nv[r] = node4[vo[r]];
+        // all loads complete here, before any store: the compiler would
+        // otherwise sink each load to its store and wait for one at a time
+#pragma unroll
+        for (int r = 0; r < kSortItems; r++)
+            asm volatile("" : "+v"(nv[r].x), "+v"(nv[r].y), "+v"(nv[r].z), "+v"(nv[r].w));
+    }
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const int dst = dsto[r];
+        if (dst < G.noff || dst >= G.noff + G.n) continue; // past the tile's end only
+        key_out[dst] = ko[r];
+        val_out[dst] = vo[r];
+        if (GATHER) psort[dst] = nv[r];
     }
 }
 
