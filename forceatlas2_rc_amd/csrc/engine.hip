@@ -679,33 +679,51 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     // graph has no region of its own index
     const int nslots = (hi == gend) ? hi - lo - 1 : hi - lo;
     const bool adjust = D.flags & GF_ADJUST;
-    for (int q = threadIdx.x; q < hi - lo; q += blockDim.x) {
-        const V4 leaf = psort[lo + q];
+    // One position per thread (a block has at most blockDim.x positions).
+    // Every load of the thread is issued before the first wait: its leaf,
+    // the escape link after it, the links of region tbase + q and the leaf's
+    // parent; indices are clamped to valid entries and unused values
+    // dropped.
+    const int q = (int) threadIdx.x;
+    const bool has_leaf = q < hi - lo, has_slot = q < nslots;
+    const int pos = lo + min(q, hi - lo - 1);
+    const int t = tbase + min(q, max(nslots - 1, 0));
+    V4 leaf = psort[pos];
+    int esc_pos = escape[min(pos, gend - 2) - g];
+    int pl = parent_leaf[pos];
+    int4 l = make_int4(0, 0, 0, 0);
+    int2 ch = make_int2(0, 0);
+    int par = -1;
+    if (nslots > 0) { // a block of the graph's last position alone has no region
+        l = tn[t].l;
+        ch = child[t];
+        par = parent_int[t];
+    }
+    asm volatile("" : "+v"(leaf.x), "+v"(leaf.y), "+v"(leaf.z), "+v"(leaf.w), "+v"(esc_pos),
+                 "+v"(pl), "+v"(l.z), "+v"(l.w), "+v"(ch.x), "+v"(ch.y), "+v"(par));
+    const bool inside = has_slot && l.z >= lo && l.w < hi;
+    // the skip link of a region merged here, loaded now for the end
+    const int esc_last = inside ? escape[min(l.w, gend - 2) - g] : 0;
+    if (has_leaf) {
         s_leaf[q] = leaf;
         // the leaf's own traversal record (NI + sorted position): a region of
         // one node and size 0. Accepted as a monopole it gives exactly the
         // pair term without sizes; with sizes it is never accepted (size^2
         // infinite) and is visited as a pair.
-        const int pos = lo + q;
         TNode<R> rec;
         rec.c = mk4<R>(leaf.x, leaf.y, leaf.z, adjust ? (R) INFINITY : (R) 0);
-        rec.l = make_int4(0, pos == gend - 1 ? kNoRecord : record_of(escape[pos - g], NI), pos,
-                          pos);
+        rec.l = make_int4(0, pos == gend - 1 ? kNoRecord : record_of(esc_pos, NI), pos, pos);
         tn[(size_t) NI + pos] = rec;
-        int vis = kOutside;
-        if (q < nslots) {
-            const int t = tbase + q;
-            const int4 l = tn[t].l;
-            if (l.z >= lo && l.w < hi) vis = 0;
-            s_child[q] = child[t];
-            s_parent[q] = parent_int[t];
+        if (has_slot) {
+            s_child[q] = ch;
+            s_parent[q] = par;
         }
-        s_vis[q] = vis;
+        s_vis[q] = inside ? 0 : kOutside;
     }
     __syncthreads();
-    for (int k = lo + (int) threadIdx.x; k < hi; k += blockDim.x) {
-        int t = parent_leaf[k];
-        Summ<R> r = leaf_summ<R>(s_leaf[k - lo]);
+    if (has_leaf) {
+        int t = pl;
+        Summ<R> r = leaf_summ<R>(leaf);
         // a root-to-leaf path has at most 64 regions (32 key bits, then up to
         // 32 bits of position); the bound only guards against a corrupt tree
         for (int depth = 0; t >= 0 && depth < 128; depth++) {
@@ -753,14 +771,8 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
             t = p;
         }
     }
-    __syncthreads();
     // the skip links of the regions merged here
-    for (int q = threadIdx.x; q < nslots; q += blockDim.x) {
-        if (s_vis[q] >= kOutside) continue;
-        const int t = tbase + q;
-        const int last = tn[t].l.w;
-        tn[t].l.y = last == gend - 1 ? kNoRecord : record_of(escape[last - g], NI);
-    }
+    if (inside) tn[t].l.y = l.w == gend - 1 ? kNoRecord : record_of(esc_last, NI);
 }
 
 // One unit of level `level` >= 2: its listed regions are staged in LDS, one
