@@ -516,6 +516,7 @@ struct Unit {
 constexpr int kKarrasThreads = 256;
 constexpr int kKarrasWin = 1024;
 constexpr int kKarrasStage = 2 * kKarrasThreads + 2 * kKarrasWin;
+static_assert(kKarrasStage % kKarrasThreads == 0, "whole rounds of staging");
 
 template <typename R>
 __global__ void __launch_bounds__(kKarrasThreads)
@@ -529,17 +530,27 @@ k_karras(int NI, int N, const int *__restrict__ gi, const GraphDesc *__restrict_
     const int t0 = (int) blockIdx.x * kKarrasThreads;
     const int t = t0 + (int) threadIdx.x;
     const int tl = min(t0 + kKarrasThreads - 1, NI - 1);
+    // the region's graph and descriptor are loaded beside the keys
+    const int g = gi[min(t, NI - 1)];
+    const bool retired = gs[g].retired;
+    const GraphDesc G = gd[g];
     const int lo = t0 + gi[t0] - kKarrasWin; // s_key[q] holds the key at position lo + q
     const int cnt = min(tl + gi[tl] + kKarrasWin + 1 - lo, kKarrasStage);
-    for (int q = (int) threadIdx.x; q < cnt; q += kKarrasThreads) {
+    constexpr int kPer = kKarrasStage / kKarrasThreads;
+    uint32_t kv[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; r++) { // every load in flight at once
+        const int p = min(max(lo + (int) threadIdx.x + r * kKarrasThreads, 0), N - 1);
+        kv[r] = ks[p];
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const int q = (int) threadIdx.x + r * kKarrasThreads;
         const int p = lo + q;
-        s_key[q] = p >= 0 && p < N ? ks[p] : 0u;
+        if (q < cnt) s_key[q] = p >= 0 && p < N ? kv[r] : 0u;
     }
     __syncthreads();
-    if (t >= NI) return;
-    const int g = gi[t];
-    if (gs[g].retired) return;
-    const GraphDesc G = gd[g];
+    if (t >= NI || retired) return;
     const int n = G.n, i = t - G.ioff, base = G.noff;
     auto key = [&](int b) -> uint32_t { // key at local position b of the graph
         const int q = base + b - lo;
