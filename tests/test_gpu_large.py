@@ -301,6 +301,29 @@ def test_theta_sweep():
     assert err(0.0, "single") < 1e-3
 
 
+def test_records_beyond_4_gib_in_double():
+    """Traversal records past 4 GiB: in double precision a record takes 64
+    bytes, so a batch of 2^25 + 3000 nodes holds more than 2^26 records and
+    the traversal indexes them in 64 bits. The small graph after the large
+    one walks records beyond 4 GiB; it must give its layout alone bit for
+    bit, and the large one (isolated nodes, uniform) finite positions."""
+    n_a = 1 << 25
+    rng = np.random.default_rng(31)
+    big = dict(n=n_a, edges=np.zeros((0, 2), np.int32), init=rng.uniform(-1e4, 1e4, (n_a, 2)),
+               iterations=2)
+    n_b, e_b = pa(3000, 3, seed=32)
+    small = dict(n=n_b, edges=e_b, init=start(n_b, seed=33), linlog=True, iterations=2)
+    with Batch([big, small], precision="double", autotune=False) as b:
+        b.run(2)
+        b.sync()
+        pos_b = b.positions(1)
+        assert np.all(np.isfinite(b.positions(0)))
+    with Batch([small], precision="double", autotune=False) as one:
+        one.run(2)
+        one.sync()
+        assert np.array_equal(one.positions(0), pos_b)
+
+
 def test_reproducible_and_same_alone_as_in_a_batch():
     """test-large.R:59-71, and batch composition invariance."""
     n, e = pa(5000)
