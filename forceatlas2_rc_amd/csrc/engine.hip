@@ -11,7 +11,9 @@
 //   pos    R2   positions used by the kernels (float32 copies in single
 //               precision, the positions themselves in double precision)
 //   acc    f64x2 position accumulators (single precision only)
-//   F[2]   R2   force of this iteration / of the previous one (ping-pong)
+//   F[2]   f64x2 force of this iteration / of the previous one (ping-pong)
+//   Fa     f64x2 the iteration's attraction (computed on the second stream
+//               beside the traversal; k_swing adds it to the force)
 //   keys/vals   Morton codes and node ids, two buffers for the radix sort
 //   psort  R4   (x, y, mass, radius) in sorted order
 //   tn     {R4 (cx, cy, mass, size^2), int4 (left, skip, first, last)} for
