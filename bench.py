@@ -26,6 +26,10 @@ Beside it, in the same line:
   * "allpairs": the exact all-pairs repulsion kernel of the offload path on
     a 50,000-node preferential-attachment graph, FP32, its FLOP/s against
     the FP32 peak (12 FLOP per ordered pair);
+  * "offload": the same size of graph laid out through the repulsion
+    offload (the host runs the iteration), exact and Barnes-Hut, each beside
+    the oracle's CPU layout with the same algorithm, and through the
+    all-GPU path;
   * "batched": 512 stochastic-block-model graphs (200..20000 nodes,
     log-uniform; 2..20 equal blocks; mean in-block degree 8, cross-block 1),
     200 iterations each, laid out as one batch and one at a time;
@@ -168,6 +172,64 @@ def allpairs_leg(args):
                 kernel="k_exact", kernel_ms=k_ms, flop_per_launch=flop, achieved=achieved,
                 peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP32_PEAK_TFLOPS,
                 bound="valu")
+
+
+def offload_leg(args):
+    """The C2 workload through the repulsion offload (forceatlas2_gpu: the
+    host runs the iteration, the GPU the repulsion): a 50,000-node
+    preferential-attachment graph (m = 3), weights uniform in [0.5, 2)
+    rounded to float32, linlog off, single precision. Exact all-pairs
+    repulsion and Barnes-Hut (theta 1.2, the Gephi quadtree built on the host
+    each iteration, the traversal on the GPU), each against the oracle's CPU
+    layout with the same algorithm in FP64 on the host's cores (the offload's
+    host part uses as many threads); and the same graph through the all-GPU
+    path. Bounded samples: --offload-iters
+    iterations on the GPU, 2 (exact) and 10 (Barnes-Hut) on the CPU, each
+    after one warm-up iteration."""
+    import forceatlas2_rc_amd as F
+    from forceatlas2_rc_amd import Batch, graphs
+    from oracle import oracle as O
+    n, e = graphs.barabasi_albert(args.allpairs_nodes, 3, seed=args.seed + 8)
+    rng = np.random.default_rng(args.seed + 9)
+    w = rng.uniform(0.5, 2.0, len(e)).astype(np.float32).astype(np.float64)
+    r = 10.0 * np.sqrt(n)
+    pos = rng.uniform(-r, r, size=(n, 2))
+    K = args.offload_iters
+    threads = min(16, os.cpu_count() or 1)
+    out = {}
+    for name, bh in (("exact", False), ("barnes_hut", True)):
+        F.layout_forceatlas2_gpu(n, e, 2, weights=w, init=pos, barnes_hut=bh, theta=1.2,
+                                 threads=threads)
+        t0 = time.perf_counter()
+        _, tm = F.layout_forceatlas2_gpu(n, e, K, weights=w, init=pos, barnes_hut=bh,
+                                         theta=1.2, threads=threads, timings=True)
+        dt = time.perf_counter() - t0
+        out[name] = dict(ms_per_iteration=dt / K * 1e3, node_iters_per_s=n * K / dt,
+                         host_threads=threads,
+                         phase_ms_per_iteration={k: v / K * 1e3 for k, v in tm.items()})
+    for name, bh, k in (("exact", False, 2), ("barnes_hut", True, 10)):
+        kw = dict(scaling=2.0, gravity=1.0, theta=1.2, barnes_hut=bh, w=w, threads=threads)
+        p1 = O.layout(n, e, pos, 1, **kw)
+        t0 = time.perf_counter()
+        O.layout(n, e, p1, k, **kw)
+        dt = time.perf_counter() - t0
+        out[name]["cpu_node_iters_per_s"] = n * k / dt
+        out[name]["cpu_sample"] = f"{k} iterations after 1 warm-up, FP64, {threads} threads"
+        out[name]["gpu_over_cpu"] = out[name]["node_iters_per_s"] / out[name]["cpu_node_iters_per_s"]
+    g = dict(n=n, edges=e, weights=w, init=pos, theta=1.2, scaling=2.0, gravity=1.0)
+    with Batch([g], autotune=False) as b:
+        b.run(2)
+        b.sync()
+        t0 = time.perf_counter()
+        b.run(K)
+        b.sync()
+        dt = time.perf_counter() - t0
+    out["all_gpu_barnes_hut"] = dict(ms_per_iteration=dt / K * 1e3, node_iters_per_s=n * K / dt)
+    log("offload:", json.dumps({k: round(v["ms_per_iteration"], 3) for k, v in out.items()}))
+    return dict(workload=(f"repulsion offload (forceatlas2_gpu), preferential attachment n={n}, "
+                          "m=3, weights U(0.5, 2) as float32, linlog off, FP32; exact and "
+                          "Barnes-Hut theta=1.2, and the all-GPU path on the same graph"),
+                nodes=n, edges=int(len(e)), iterations=K, **out)
 
 
 def batched_leg(args):
@@ -364,6 +426,8 @@ def main():
     ap.add_argument("--no-allpairs", action="store_true")
     ap.add_argument("--allpairs-nodes", type=int, default=50_000)
     ap.add_argument("--allpairs-calls", type=int, default=10)
+    ap.add_argument("--no-offload", action="store_true")
+    ap.add_argument("--offload-iters", type=int, default=20)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--batch-iters", type=int, default=200)
@@ -511,6 +575,8 @@ def main():
     del graph, mass, off, node, coef
     if not args.no_allpairs:
         out["allpairs"] = allpairs_leg(args)
+    if not args.no_offload:
+        out["offload"] = offload_leg(args)
     if not args.no_batched:
         out["batched"] = batched_leg(args)
     if not args.no_serving:
