@@ -7,11 +7,10 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "message.h"
+
 namespace fa2 {
 
-// Thread-local error message (the reference keeps one static buffer,
-// src/gpu.c:129-143; here each host thread has its own).
-void set_message(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 void set_hip_error(const char *what, hipError_t e);
 
 // Stream of a device, created on first use and kept until

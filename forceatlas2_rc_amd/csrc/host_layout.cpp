@@ -12,7 +12,7 @@
 #include <cstring>
 #include <vector>
 
-#include "common.h"
+#include "message.h"
 #include "../../include/fa2.h"
 #include "../../include/fa2_hip.h"
 
@@ -78,7 +78,10 @@ struct Level {
 extern "C" void fa2_tree_build(fa2_tree *t, int n, const double *x, const double *y,
                                const double *mass, const double *size, int threads)
 {
-    (void) threads;
+    // the regions of a level are measured and split independently: with
+    // threads > 1 they are shared among threads (src/forceatlas2.c:108, :169,
+    // :225); region ids and node order do not depend on the thread count
+    const int nt = threads > 1 ? threads : 1;
     fa2_region *reg = t->reg;
     // root: every node, centre of mass (mass[0] and the node itself if alone)
     double msum = 0.0, mx = 0.0, my = 0.0;
@@ -93,7 +96,12 @@ extern "C" void fa2_tree_build(fa2_tree *t, int n, const double *x, const double
     t->nreg = 1;
     Level L{t};
     for (int lo = 0, hi = 1; lo < hi; lo = hi, hi = t->nreg) {
+        // a level is shared among threads when it has regions for all of
+        // them: the top levels hold few large regions, whose split stays on
+        // one thread (measured faster than a parallel loop over 1..16 regions)
+        const bool wide = nt > 1 && hi - lo >= 4 * nt;
         // sizes and quadrant populations
+#pragma omp parallel for schedule(static) num_threads(nt) if (wide)
         for (int r = lo; r < hi; r++) {
             int *q = L.quad(r, lo);
             q[0] = q[1] = q[2] = q[3] = 0;
@@ -137,6 +145,7 @@ extern "C" void fa2_tree_build(fa2_tree *t, int n, const double *x, const double
             }
         }
         // move the nodes to their quadrant; masses and centres of the children
+#pragma omp parallel for schedule(static) num_threads(nt) if (wide)
         for (int r = lo; r < hi; r++) {
             const fa2_region &R = reg[r];
             if (R.nchild == 0) continue;
@@ -178,7 +187,9 @@ extern "C" void fa2_tree_build(fa2_tree *t, int n, const double *x, const double
         }
     }
     // the nodes in tree order, from the leaf regions
-    for (int r = 0; r < t->nreg; r++) {
+    const int nreg = t->nreg;
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
+    for (int r = 0; r < nreg; r++) {
         if (reg[r].nchild > 0) continue;
         const fa2_node *nd = t->buf[reg[r].buf];
         for (int k = reg[r].start; k < reg[r].start + reg[r].count; k++) {
@@ -277,6 +288,46 @@ extern "C" int fa2_layout_run(int n, int m, const int *from, const int *to, cons
     mean_mass /= n;
     const double coef = outbound ? mean_mass : 1.0;
 
+    // With threads > 1 the host loops are shared among threads, node by node
+    // (src/forceatlas2.c:607-631, :720-802): the attraction walks each
+    // node's incidences in edge order and repeats the edge loop's arithmetic
+    // for each of them, so every node receives the same terms in the same
+    // order and the layout does not depend on the thread count.
+    const int nt = threads > 1 ? threads : 1;
+    std::vector<int> inc_off, inc_edge;
+    if (nt > 1) {
+        inc_off.assign(nn + 1, 0);
+        for (int e = 0; e < m; e++)
+            if (from[e] != to[e]) {
+                inc_off[(size_t) from[e] + 1]++;
+                inc_off[(size_t) to[e] + 1]++;
+            }
+        for (size_t i = 0; i < nn; i++) inc_off[i + 1] += inc_off[i];
+        inc_edge.resize((size_t) inc_off[nn]);
+        std::vector<int> at(inc_off.begin(), inc_off.end() - 1);
+        for (int e = 0; e < m; e++)
+            if (from[e] != to[e]) {
+                inc_edge[(size_t) at[from[e]]++] = e;
+                inc_edge[(size_t) at[to[e]]++] = e;
+            }
+    }
+    // the attraction of edge e (:385-416), from its first node's side: false
+    // when it adds nothing
+    auto edge_term = [&](int e, double &xd, double &yd, double &f) {
+        const int a = from[e], b = to[e];
+        xd = x[a] - x[b];
+        yd = y[a] - y[b];
+        f = -coef * w[e];
+        if (adjust || linlog) {
+            double d = std::sqrt(xd * xd + yd * yd);
+            if (adjust) d -= size[a] + size[b];
+            if (!(d > 0)) return false;
+            if (linlog) f *= std::log1p(d) / d;
+        }
+        if (outbound) f /= mass[a];
+        return f != 0.0;
+    };
+
     HostTree tree;
     if (barnes_hut) tree.alloc(n, adjust != 0);
     fa2_gpu *g = fa2_gpu_open(device, gpu[1], n, adjust, barnes_hut, threads);
@@ -304,7 +355,8 @@ extern "C" int fa2_layout_run(int n, int m, const int *from, const int *to, cons
         t_rep += fa2_now() - t0;
         if (status) break;
         // gravity (:371-381)
-        if (gravity != 0)
+        if (gravity != 0) {
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
             for (int i = 0; i < n; i++) {
                 const double d = std::sqrt(x[i] * x[i] + y[i] * y[i]);
                 if (d > 0) {
@@ -313,27 +365,39 @@ extern "C" int fa2_layout_run(int n, int m, const int *from, const int *to, cons
                     fy[i] -= y[i] * f;
                 }
             }
-        // attraction, edge by edge (:385-416)
-        for (int e = 0; e < m; e++) {
-            const int a = from[e], b = to[e];
-            if (a == b) continue;
-            const double xd = x[a] - x[b], yd = y[a] - y[b];
-            double f = -coef * w[e];
-            if (adjust || linlog) {
-                double d = std::sqrt(xd * xd + yd * yd);
-                if (adjust) d -= size[a] + size[b];
-                if (!(d > 0)) continue;
-                if (linlog) f *= std::log1p(d) / d;
+        }
+        // attraction (:385-416): edge by edge, or node by node over the
+        // incidences in edge order
+        if (nt > 1) {
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt)
+            for (int i = 0; i < n; i++) {
+                for (int k = inc_off[(size_t) i]; k < inc_off[(size_t) i + 1]; k++) {
+                    const int e = inc_edge[(size_t) k];
+                    double xd, yd, f;
+                    if (!edge_term(e, xd, yd, f)) continue;
+                    if (from[e] == i) {
+                        fx[i] += xd * f;
+                        fy[i] += yd * f;
+                    } else {
+                        fx[i] -= xd * f;
+                        fy[i] -= yd * f;
+                    }
+                }
             }
-            if (outbound) f /= mass[a];
-            if (f == 0.0) continue;
-            fx[a] += xd * f;
-            fy[a] += yd * f;
-            fx[b] -= xd * f;
-            fy[b] -= yd * f;
+        } else {
+            for (int e = 0; e < m; e++) {
+                if (from[e] == to[e]) continue;
+                double xd, yd, f;
+                if (!edge_term(e, xd, yd, f)) continue;
+                fx[from[e]] += xd * f;
+                fy[from[e]] += yd * f;
+                fx[to[e]] -= xd * f;
+                fy[to[e]] -= yd * f;
+            }
         }
         // swinging and traction, summed in node order (:758-778)
         double tsw = 0.0, ttr = 0.0;
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
         for (int i = 0; i < n; i++) {
             const double sx = ox[i] - fx[i], sy = oy[i] - fy[i];
             const double tx = ox[i] + fx[i], ty = oy[i] + fy[i];
@@ -347,6 +411,7 @@ extern "C" int fa2_layout_run(int n, int m, const int *from, const int *to, cons
         }
         fa2_update_speed(n, tsw, ttr, jt_user, &speed, &eff);
         // displacement (:780-802)
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
         for (int i = 0; i < n; i++) {
             if (fixed && fixed[i]) continue;
             double factor;

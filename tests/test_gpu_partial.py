@@ -142,6 +142,31 @@ def test_double_layout_equals_cpu(bh):
         assert mean_rel(got, want) < 1e-9, list(kw)
 
 
+@pytest.mark.parametrize("bh", [False, True])
+def test_layout_does_not_depend_on_host_threads(bh):
+    """test-algorithm.R:88-103 for the offload: with threads > 1 the host
+    runs the tree build and the attraction, swing and displacement loops on
+    several threads (the attraction node by node over incidence lists in
+    edge order); the layout is bit-identical to one thread's, over the
+    option sets, with self-loops and a repeated edge."""
+    n, e = graphs.barabasi_albert(3000, 3, seed=44)
+    e = np.concatenate([e, [[5, 5], [7, 9], [7, 9]]])
+    pos = graphs.uniform_positions(n, -200, 200, 45)
+    rng = np.random.default_rng(46)
+    w = rng.uniform(0.2, 3, len(e))
+    size = rng.uniform(0.5, 4, n)
+    sets = [dict(), dict(linlog=True, dissuade_hubs=True, weights=w),
+            dict(prevent_overlap=True, node_size=size, strong_gravity=True),
+            dict(fixed=np.arange(n) % 3 == 0, gravity=2.0)]
+    for kw in sets:
+        for precision in ("single", "double"):
+            one = F.layout_forceatlas2_gpu(n, e, 8, init=pos, barnes_hut=bh, threads=1,
+                                           precision=precision, **kw)
+            many = F.layout_forceatlas2_gpu(n, e, 8, init=pos, barnes_hut=bh, threads=6,
+                                            precision=precision, **kw)
+            assert np.array_equal(one, many), (list(kw), precision)
+
+
 def test_single_layout_close_to_cpu():
     """test-gpu.R:60-79."""
     n, e = graphs.barabasi_albert(400, 2, seed=42)
