@@ -1945,7 +1945,11 @@ void EngineImpl<R>::launch_attract_class(int c, int u, double2 *Fa, hipStream_t 
 
 template <typename R> void EngineImpl<R>::launch_attraction(hipStream_t st)
 {
-    for (int c = 0; c < 3; c++) launch_attract_class(c, shape.attract_unroll[c], d_Fa.p, st);
+    // the hubs' work items first: the largest kernel gets the wave slots that
+    // free up first beside the traversal, and the lists of few incidences
+    // fill in after it (each kernel writes its own nodes: the order changes
+    // no result)
+    for (int c = 2; c >= 0; c--) launch_attract_class(c, shape.attract_unroll[c], d_Fa.p, st);
     if (!h_hubs.empty())
         k_attract_fix<R><<<ceil_div((int) h_hubs.size(), 4), 256, 0, st>>>(
             (int) h_hubs.size(), d_hubs.p, d_gid.p, d_gs.p, d_hubpart.p, d_Fa.p);
