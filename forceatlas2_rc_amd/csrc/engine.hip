@@ -966,7 +966,7 @@ __device__ inline void lane_pair_adjust(typename Vec<R>::v4 me, typename Vec<R>:
     const double dx = (double) (me.x - o.x), dy = (double) (me.y - o.y);
     const double d = __builtin_sqrt(dx * dx + dy * dy) - ((double) me.w + (double) o.w);
     const double mo = (double) o.z;
-    double f = d > 0.0 ? mo / (d * d) : (d < 0.0 ? 100.0 * mo : 0.0);
+    double f = d > 0.0 ? mo * frcp_r(d * d) : (d < 0.0 ? 100.0 * mo : 0.0);
     f = on ? f : 0.0;
     sum.x += dx * f;
     sum.y += dy * f;
@@ -1026,9 +1026,10 @@ k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restric
                                             (uint32_t) c * (uint32_t) sizeof(TNode<R>));
             const V4 rc = node.c;
             const int4 rl = node.l;
-            if (sizeof(R) == 4) // keep the eight dwords together in SGPRs
-                asm volatile("" ::"s"(rc.x), "s"(rc.y), "s"(rc.z), "s"(rc.w), "s"(rl.x), "s"(rl.y),
-                             "s"(rl.z), "s"(rl.w));
+            // keep the record together in SGPRs: one load, one wait (else the
+            // compiler loads the mass apart, inside the branch that uses it)
+            asm volatile("" ::"s"(rc.x), "s"(rc.y), "s"(rc.z), "s"(rc.w), "s"(rl.x), "s"(rl.y),
+                         "s"(rl.z), "s"(rl.w));
             const R xd = me.x - rc.x, yd = me.y - rc.y;
             const R d2 = xd * xd + yd * yd;
             const uint64_t am = __builtin_amdgcn_ballot_w64(rl.z > done);
