@@ -242,14 +242,46 @@ k_sort_upsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
     hist[(size_t) g.toff * 256 + (size_t) threadIdx.x * g.ntiles + t.j] = h[threadIdx.x];
 }
 
-// One wave per (graph, digit) row: exclusive scan in place, row total out.
+// The tile counts of a sort pass hold one row of ntiles counts per (graph,
+// digit); each row is scanned (exclusive, in place) and its total written to
+// rowtot[256 g + digit]. A graph's rows go to the kernel for its size, over a
+// list of such graphs (glist): one thread per row up to kRowThreadTiles
+// tiles, one wave per row up to 64, one workgroup per row beyond. (One wave
+// for a row of a few tiles would make a batch of many small graphs launch
+// 256 mostly idle waves per graph.)
+constexpr int kRowThreadTiles = 16;
+
 __global__ void __launch_bounds__(256)
-k_sort_rowscan(int rows, const GraphDesc *__restrict__ gd, const GraphState *__restrict__ gs,
-               uint32_t *__restrict__ hist, uint32_t *__restrict__ rowtot)
+k_sort_rowscan_thread(int rows, const int *__restrict__ glist, const GraphDesc *__restrict__ gd,
+                      const GraphState *__restrict__ gs, uint32_t *__restrict__ hist,
+                      uint32_t *__restrict__ rowtot)
+{
+    const int r = blockIdx.x * 256 + (int) threadIdx.x;
+    if (r >= rows) return;
+    const int g = glist[r >> 8], d = r & 255;
+    if (gs[g].retired) return;
+    const GraphDesc G = gd[g];
+    uint32_t *row = hist + (size_t) G.toff * 256 + (size_t) d * G.ntiles;
+    uint32_t v[kRowThreadTiles];
+#pragma unroll
+    for (int j = 0; j < kRowThreadTiles; j++) v[j] = j < G.ntiles ? row[j] : 0u;
+    uint32_t carry = 0;
+#pragma unroll
+    for (int j = 0; j < kRowThreadTiles; j++) {
+        if (j < G.ntiles) row[j] = carry;
+        carry += v[j];
+    }
+    rowtot[g * 256 + d] = carry;
+}
+
+__global__ void __launch_bounds__(256)
+k_sort_rowscan(int rows, const int *__restrict__ glist, const GraphDesc *__restrict__ gd,
+               const GraphState *__restrict__ gs, uint32_t *__restrict__ hist,
+               uint32_t *__restrict__ rowtot)
 {
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= rows) return;
-    const int g = r >> 8, d = r & 255;
+    const int g = glist[r >> 8], d = r & 255;
     if (gs[g].retired) return;
     const GraphDesc G = gd[g];
     uint32_t *row = hist + (size_t) G.toff * 256 + (size_t) d * G.ntiles;
@@ -267,17 +299,17 @@ k_sort_rowscan(int rows, const GraphDesc *__restrict__ gd, const GraphState *__r
         if (j < G.ntiles) row[j] = carry + s - v;
         carry += __shfl(s, 63, 64);
     }
-    if (lane == 0) rowtot[r] = carry;
+    if (lane == 0) rowtot[g * 256 + d] = carry;
 }
 
-// The same with one block per row, for graphs of more than 64 tiles.
 __global__ void __launch_bounds__(256)
-k_sort_rowscan_block(const GraphDesc *__restrict__ gd, const GraphState *__restrict__ gs,
-                     uint32_t *__restrict__ hist, uint32_t *__restrict__ rowtot)
+k_sort_rowscan_block(const int *__restrict__ glist, const GraphDesc *__restrict__ gd,
+                     const GraphState *__restrict__ gs, uint32_t *__restrict__ hist,
+                     uint32_t *__restrict__ rowtot)
 {
     __shared__ uint32_t wsum[4];
     const int r = blockIdx.x;
-    const int g = r >> 8, d = r & 255;
+    const int g = glist[r >> 8], d = r & 255;
     if (gs[g].retired) return;
     const GraphDesc G = gd[g];
     uint32_t *row = hist + (size_t) G.toff * 256 + (size_t) d * G.ntiles;
@@ -301,7 +333,7 @@ k_sort_rowscan_block(const GraphDesc *__restrict__ gd, const GraphState *__restr
         __syncthreads();
         carry += total;
     }
-    if (threadIdx.x == 0) rowtot[r] = carry;
+    if (threadIdx.x == 0) rowtot[g * 256 + d] = carry;
 }
 
 // Stable scatter of one tile. Wave w ranks its 512 keys in 8 rounds of 64
@@ -1496,7 +1528,10 @@ template <typename R> struct EngineImpl final : Engine {
     DevBuf<int> d_gid, d_gi;
     DevBuf<V2> d_pos;
     DevBuf<V4> d_node4; // (x, y, mass, radius) in the original order
-    int max_tiles = 1;
+    // the graphs whose tile-count rows each rowscan kernel scans: up to
+    // kRowThreadTiles tiles, up to 64, more
+    std::vector<int> h_rowg[3];
+    DevBuf<int> d_rowg[3];
     DevBuf<double2> d_F[2]; // forces are kept in FP64: the swing is their difference
     // the attraction of the iteration, computed beside the repulsion on a
     // second stream and added to it by k_swing (zero for nodes without edges)
@@ -1678,7 +1713,10 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         toff += D.ntiles;
     }
     T = toff;
-    for (const GraphDesc &D : h_gd) max_tiles = std::max(max_tiles, D.ntiles);
+    for (int g = 0; g < G; g++) {
+        const int nt = h_gd[g].ntiles;
+        h_rowg[nt <= kRowThreadTiles ? 0 : (nt <= 64 ? 1 : 2)].push_back(g);
+    }
     // summary units: a graph takes part in level L while it is longer than
     // a unit of level L - 1; its last level has a single unit
     std::vector<int> ubase((size_t) G * kMaxLevels, -1);
@@ -1828,6 +1866,7 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     hs.check(d_valB.alloc(nn, ast), "alloc");
     hs.check(d_hist.alloc(256 * (size_t) T, ast), "alloc");
     hs.check(d_rowtot.alloc(256 * (size_t) G, ast), "alloc");
+    for (int c = 0; c < 3; c++) hs.check(d_rowg[c].alloc(h_rowg[c].size(), ast), "alloc");
     hs.check(d_child.alloc(NI, ast), "alloc");
     hs.check(d_tn.alloc((size_t) NI + nn, ast), "alloc"); // regions, then one record per leaf
     hs.check(d_parent_leaf.alloc(nn, ast), "alloc");
@@ -1879,6 +1918,7 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     for (int c = 0; c < 2; c++) up(d_class[c].p, h_class[c].data(), h_class[c].size() * sizeof(int4));
     up(d_items.p, h_items.data(), h_items.size() * sizeof(int4));
     up(d_hubs.p, h_hubs.data(), h_hubs.size() * sizeof(int4));
+    for (int c = 0; c < 3; c++) up(d_rowg[c].p, h_rowg[c].data(), h_rowg[c].size() * sizeof(int));
     if (hs.ok())
         hs.check(hipMemsetAsync(d_psort.p + nn, 0, kLeafSize * sizeof(V4), stream), "memset");
     if (hs.ok()) hs.check(hipMemsetAsync(d_F[0].p, 0, nn * sizeof(double2), stream), "memset");
@@ -2019,11 +2059,15 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
         uint32_t *ko = pass % 2 ? d_keyA.p : d_keyB.p, *vo = pass % 2 ? d_valA.p : d_valB.p;
         k_sort_upsweep<<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, ki, 8 * pass,
                                                         d_hist.p);
-        if (max_tiles > 64)
-            k_sort_rowscan_block<<<256 * G, 256, 0, stream>>>(d_gd.p, d_gs.p, d_hist.p, d_rowtot.p);
-        else
-            k_sort_rowscan<<<ceil_div(256 * (size_t) G, 4), 256, 0, stream>>>(
-                256 * G, d_gd.p, d_gs.p, d_hist.p, d_rowtot.p);
+        if (const int n0 = (int) h_rowg[0].size())
+            k_sort_rowscan_thread<<<n0, 256, 0, stream>>>(256 * n0, d_rowg[0].p, d_gd.p, d_gs.p,
+                                                          d_hist.p, d_rowtot.p);
+        if (const int n1 = (int) h_rowg[1].size())
+            k_sort_rowscan<<<64 * n1, 256, 0, stream>>>(256 * n1, d_rowg[1].p, d_gd.p, d_gs.p,
+                                                        d_hist.p, d_rowtot.p);
+        if (const int n2 = (int) h_rowg[2].size())
+            k_sort_rowscan_block<<<256 * n2, 256, 0, stream>>>(d_rowg[2].p, d_gd.p, d_gs.p,
+                                                               d_hist.p, d_rowtot.p);
         if (pass < 3)
             k_sort_downsweep<V4, false><<<T, kSortThreads, 0, stream>>>(
                 d_tiles.p, d_gd.p, d_gs.p, ki, vi, 8 * pass, d_hist.p, d_rowtot.p, ko, vo, nullptr,
