@@ -560,7 +560,8 @@ def main():
                                attract_lane_unroll_1_2_4_8=shape_ms[10:14].tolist(),
                                attract_16lane_unroll_1_2_4_8=shape_ms[14:18].tolist(),
                                attract_wave_unroll_1_2_4_8=shape_ms[18:22].tolist(),
-                               attract_fork_0_1_2_3=shape_ms[22:26].tolist())),
+                               attract_fork_0_1_2_3=shape_ms[22:26].tolist(),
+                               beside_attraction_threads_chosen_256=shape_ms[26:28].tolist())),
                            stage_ms={k: v * 1e3 for k, v in stages.items()},
                            profiled_ms_per_step=dt_prof / args.steps * 1e3,
                            replay_equals_eager=bool(np.array_equal(pos_replay, pos_out)),

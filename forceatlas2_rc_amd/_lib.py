@@ -61,7 +61,7 @@ class Tuning(Structure):
                 ("repulse_xcd_run", c_int), ("attract_fork", c_int)]
 
 
-FA2_TUNE_NTIMES = 26
+FA2_TUNE_NTIMES = 28
 
 
 class FullParams(Structure):

@@ -406,7 +406,9 @@ class Batch:
         the fastest (fa2_batch_tune). Returns the chosen shapes and the
         timings in ms (traversal at 64, 128, 256, 512, 1024 threads; at XCD runs of 0,
         8, 32, 128, 512 blocks; unrolls 1, 2, 4, 8 of each attraction class; then
-        phases 1-6 of an iteration with the attraction forked at 0, 1, 2, 3),
+        phases 1-6 of an iteration with the attraction forked at 0, 1, 2, 3, then
+        phases 1-6 with the chosen traversal workgroup size and with 256 threads,
+        0 when 256 was chosen),
         or None for timings when a cached choice was reused."""
         t = Tuning()
         times = np.full(FA2_TUNE_NTIMES, -1.0)

@@ -1243,6 +1243,15 @@ template <int GL> __device__ inline void group_sum(double &x, double &y)
     }
 }
 
+// The attraction's workgroups go to the XCDs in runs of kAttractXcdRun
+// consecutive blocks (xcd_block): the nodes of a run are neighbours in the
+// work lists, mostly of the same graph in a batch, so that one L2 holds the
+// positions their incidences gather (512 SBM graphs of 200-20000 nodes: the
+// one-lane class 0.162 -> 0.152 ms; no change on one R-MAT graph of 2^21
+// nodes, whose neighbours are random). The mapping is a bijection: no result
+// changes.
+constexpr int kAttractXcdRun = 32;
+
 // GL lanes per node; the work list holds (node, first incidence, end, graph).
 template <typename R, int GL, int U>
 __global__ void __launch_bounds__(256)
@@ -1251,7 +1260,8 @@ k_attract(int count, const int4 *__restrict__ work, const GraphDesc *__restrict_
           const R *__restrict__ size, const int *__restrict__ inc_node,
           const R *__restrict__ inc_coef, double2 *__restrict__ Fa)
 {
-    const int slot = blockIdx.x * (256 / GL) + (int) threadIdx.x / GL;
+    const int blk = xcd_block((int) blockIdx.x, (int) gridDim.x, kAttractXcdRun);
+    const int slot = blk * (256 / GL) + (int) threadIdx.x / GL;
     const int sub = (int) threadIdx.x % GL;
     const int4 wk = work[min(slot, count - 1)];
     const int i = wk.x;
@@ -1275,7 +1285,7 @@ k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict
                 const R *__restrict__ inc_coef, double2 *__restrict__ Fa,
                 double2 *__restrict__ part)
 {
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int w = xcd_block((int) blockIdx.x, (int) gridDim.x, kAttractXcdRun) * 4 + (threadIdx.x >> 6);
     if (w >= count) return;
     const int lane = lane_id();
     const int4 it = items[w];
@@ -2649,8 +2659,19 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
         const fa2_tuning keep = shape;
         shape = pick;
         for (int f = 0; f < 4; f++) t[22 + f] = best_of_3([&] { enqueue_forces(Fs, f); });
-        shape = keep;
         pick.attract_fork = choose(t + 22, 4, 2);
+        // the traversal's workgroup size was timed alone; beside the
+        // attraction a larger one can lose what it gained (512 SBM graphs:
+        // 512 threads 2.4% faster alone, phases 1-6 8% slower). Another size
+        // than the default is kept only when phases 1-6 are 2% faster too.
+        if (pick.repulse_threads != 256) {
+            shape = pick;
+            t[26] = best_of_3([&] { enqueue_forces(Fs, pick.attract_fork); });
+            shape.repulse_threads = 256;
+            t[27] = best_of_3([&] { enqueue_forces(Fs, pick.attract_fork); });
+            if (!(t[26] < 0.98 * t[27])) pick.repulse_threads = 256;
+        }
+        shape = keep;
     }
     if (e0) (void) hipEventDestroy(e0);
     if (e1) (void) hipEventDestroy(e1);

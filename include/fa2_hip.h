@@ -173,8 +173,11 @@ typedef struct fa2_tuning {
  * 1024 threads, then at XCD runs of 0, 8, 32, 128 and 512 blocks, then for
  * each attraction class the unrolls 1, 2, 4 and 8 (ms; 0 where a class is
  * empty), then phases 1-6 of an iteration (codes, sort, tree, traversal and
- * attraction) with the attraction forked at 0, 1, 2 and 3. */
-#define FA2_TUNE_NTIMES 26
+ * attraction) with the attraction forked at 0, 1, 2 and 3, then phases 1-6
+ * with the chosen traversal workgroup size and with 256 threads (0 when the
+ * chosen size is 256: a size other than 256, timed faster alone, is kept
+ * only when these phases are also 2% faster). */
+#define FA2_TUNE_NTIMES 28
 
 /* Autotuner: builds the tree of the batch's current positions and times
  * every candidate launch shape on it, with HIP events on the batch's stream
