@@ -61,7 +61,9 @@ constexpr int kCountStride = 32;
 // Device descriptors
 // ---------------------------------------------------------------------------
 
-enum : int { GF_STRONG = 1, GF_LINLOG = 2, GF_ADJUST = 4, GF_FIXED = 8 };
+// GF_UNIFORM: every node's incidences carry one coefficient (equal weights
+// without dissuade_hubs), so the attraction reads it once per node
+enum : int { GF_STRONG = 1, GF_LINLOG = 2, GF_ADJUST = 4, GF_FIXED = 8, GF_UNIFORM = 16 };
 
 struct GraphDesc {
     int noff, n, ioff, toff;
@@ -1203,13 +1205,14 @@ __device__ inline void attract_term(typename Vec<R>::v2 pi, typename Vec<R>::v2 
 // The incidences e0, e0 + step, ... < e1 of one node, in that order. Each
 // round first loads the list entries of U incidences, then their U
 // positions, then adds their terms; the index of a slot past e1 is clamped
-// to e1 - 1 (a valid entry), so every load is unconditional.
-template <typename R, int U>
+// to e1 - 1 (a valid entry), so every load is unconditional. With UNIFORM
+// every coefficient of the node equals c0 and none is loaded.
+template <typename R, int U, bool UNIFORM>
 __device__ inline void attract_terms(int e0, int e1, int step, bool linlog, bool adjust,
                                      typename Vec<R>::v2 pi, R si,
                                      const typename Vec<R>::v2 *__restrict__ pos,
                                      const R *__restrict__ size, const int *__restrict__ inc_node,
-                                     const R *__restrict__ inc_coef, ForceSum<R> &sum)
+                                     const R *__restrict__ inc_coef, R c0, ForceSum<R> &sum)
 {
     const int last = e1 - 1;
     for (int e = e0; e < e1; e += U * step) {
@@ -1219,7 +1222,7 @@ __device__ inline void attract_terms(int e0, int e1, int step, bool linlog, bool
         for (int u = 0; u < U; u++) {
             const int q = min(e + u * step, last);
             j[u] = inc_node[q];
-            c[u] = inc_coef[q];
+            c[u] = UNIFORM ? c0 : inc_coef[q];
         }
         typename Vec<R>::v2 pj[U];
         R sj[U];
@@ -1268,8 +1271,13 @@ k_attract(int count, const int4 *__restrict__ work, const GraphDesc *__restrict_
     const int flags = gd[wk.w].flags;
     const bool live = slot < count && !gs[wk.w].retired;
     ForceSum<R> sum;
-    attract_terms<R, U>(wk.y + sub, wk.z, GL, flags & GF_LINLOG, flags & GF_ADJUST, pos[i],
-                        size[i], pos, size, inc_node, inc_coef, sum);
+    if (flags & GF_UNIFORM)
+        attract_terms<R, U, true>(wk.y + sub, wk.z, GL, flags & GF_LINLOG, flags & GF_ADJUST,
+                                  pos[i], size[i], pos, size, inc_node, inc_coef, inc_coef[wk.y],
+                                  sum);
+    else
+        attract_terms<R, U, false>(wk.y + sub, wk.z, GL, flags & GF_LINLOG, flags & GF_ADJUST,
+                                   pos[i], size[i], pos, size, inc_node, inc_coef, (R) 0, sum);
     group_sum<GL>(sum.x, sum.y);
     if (!live || sub != 0) return;
     Fa[i] = make_double2(sum.x, sum.y);
@@ -1293,8 +1301,13 @@ k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict
     const int g = gid[i];
     const int flags = gd[g].flags;
     ForceSum<R> sum;
-    attract_terms<R, U>(it.y + lane, it.z, 64, flags & GF_LINLOG, flags & GF_ADJUST, pos[i],
-                        size[i], pos, size, inc_node, inc_coef, sum);
+    if (flags & GF_UNIFORM)
+        attract_terms<R, U, true>(it.y + lane, it.z, 64, flags & GF_LINLOG, flags & GF_ADJUST,
+                                  pos[i], size[i], pos, size, inc_node, inc_coef, inc_coef[it.y],
+                                  sum);
+    else
+        attract_terms<R, U, false>(it.y + lane, it.z, 64, flags & GF_LINLOG, flags & GF_ADJUST,
+                                   pos[i], size[i], pos, size, inc_node, inc_coef, (R) 0, sum);
     group_sum<64>(sum.x, sum.y);
     if (lane != 0 || gs[g].retired) return;
     if (it.w < 0) {
@@ -1771,15 +1784,27 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         for (int i = 0; i < D.n - 1; i++) gi[(size_t) D.ioff + i] = g;
         if (from_edges) continue;
         const int ni = p->inc_off[p->n];
-        for (int e = 0; e < ni; e++) {
-            const int j = p->inc_node[e];
-            if (j < 0 || j >= p->n) {
-                set_message("edge refers to a node that does not exist");
+        bool uniform = true; // every coefficient of a node equal to its first, bit for bit
+        for (int i = 0; i < p->n; i++) {
+            const int e0 = p->inc_off[i], e1 = p->inc_off[i + 1];
+            if (e0 < 0 || e1 < e0 || e1 > ni) {
+                set_message("the incidence offsets must rise from 0 to the number of incidences");
                 return 1;
             }
-            inc_node[(size_t) ioff + e] = D.noff + j;
-            coef[(size_t) ioff + e] = (R) p->inc_coef[e];
+            const R first = e1 > e0 ? (R) p->inc_coef[e0] : (R) 0;
+            for (int e = e0; e < e1; e++) {
+                const int j = p->inc_node[e];
+                if (j < 0 || j >= p->n) {
+                    set_message("edge refers to a node that does not exist");
+                    return 1;
+                }
+                inc_node[(size_t) ioff + e] = D.noff + j;
+                const R c = (R) p->inc_coef[e];
+                coef[(size_t) ioff + e] = c;
+                uniform = uniform && memcmp(&c, &first, sizeof(R)) == 0;
+            }
         }
+        if (uniform) h_gd[g].flags |= GF_UNIFORM;
         ioff += ni;
     }
     inc_off[nn] = (int) ioff;
@@ -1798,6 +1823,14 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
                                 E->outbound ? (double) (n + 2LL * E->m) / (double) n : 1.0};
             ed[g] = E;
             eo += E->m;
+            // one weight for all edges and no dissuade_hubs: a node's
+            // coefficients are all equal (with dissuade_hubs both incidences
+            // of an edge divide by the mass of its first node, so a list's
+            // coefficients vary with the other end)
+            bool same = !E->outbound;
+            for (int e = 1; E->w && e < E->m && same; e++)
+                same = memcmp(&E->w[e], &E->w[0], sizeof(double)) == 0;
+            if (same) h_gd[g].flags |= GF_UNIFORM;
         }
         if (ingest_graphs<R>(stream, ast, N, ig, ed, d_mass, d_inc_off, d_inc_node, d_coef, inc_off))
             return 1;

@@ -411,3 +411,40 @@ def test_profile_times_every_phase():
     assert t["total"] > 0
     for k in ("codes", "sort", "tree", "repulsion", "attraction", "move"):
         assert t[k] > 0, k
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_uniform_coefficients_read_once_per_node(precision):
+    """A graph whose nodes each carry one coefficient (unweighted, without
+    dissuade_hubs, whose coefficient varies along a list with the other end
+    of each edge) reads it once per node. Against the per-incidence reads,
+    forced by changing one coefficient of one node p (the graph is then not
+    uniform), the first iteration's force of every other node is
+    bit-identical: its own terms did not change. Degrees span every
+    attraction class, linlog on; the device's preprocessing gives the same
+    forces as the host's."""
+    from forceatlas2_rc_amd.api import prepare_large
+    n, e = graphs.rmat(13, 60_000, seed=21)
+    pos = start(n, seed=22, r=10 * np.sqrt(n))
+    mass, off, node, coef = prepare_large(n, e[:, 0], e[:, 1], np.ones(len(e)), False)
+    deg = np.diff(off)
+    p = int(np.argmax(deg))  # a hub: its incidences are work items
+    assert deg[p] > 1024 and (deg > 0).sum() > 100 and ((deg > 16) & (deg <= 256)).any()
+    coef2 = coef.copy()
+    coef2[off[p] + 1] *= 1.5
+    base = dict(n=n, frm=e[:, 0], to=e[:, 1], mass=mass, inc_off=off, inc_node=node,
+                init=pos, theta=1.2, linlog=True)
+    forces = []
+    for c in (coef, coef2):
+        with Batch([dict(base, inc_coef=c)], precision=precision, autotune=False) as b:
+            b.run(1)
+            b.sync()
+            forces.append(b.forces(0))
+    others = np.arange(n) != p
+    assert np.array_equal(forces[0][others], forces[1][others])
+    assert not np.array_equal(forces[0][p], forces[1][p])
+    with Batch([dict(n=n, edges=e, init=pos, theta=1.2, linlog=True)],
+               precision=precision, autotune=False, ingest="device") as b:
+        b.run(1)
+        b.sync()
+        assert np.array_equal(b.forces(0), forces[0])
