@@ -64,25 +64,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_graph(scale, n_edges, seed):
-    from forceatlas2_rc_amd import graphs
-    cache = f"/tmp/fa2_rmat_{scale}_{n_edges}_{seed}.npy"
-    if os.path.exists(cache):
-        e = np.load(cache)
-        return 1 << scale, e
-    n, e = graphs.rmat(scale, n_edges, seed=seed)
+def host_threads():
+    """Threads for the CPU baseline: every core this process may run on,
+    within the OpenMP thread count the host grants it (OMP_NUM_THREADS, when
+    set). Returns the count and what it was taken from."""
+    cpus = os.cpu_count() or 1
     try:
-        np.save(cache, e)
-    except OSError:
-        pass
-    return n, e
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = cpus
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    omp = int(omp) if omp.isdigit() and int(omp) > 0 else None
+    threads = min(allowed, omp) if omp else allowed
+    return threads, dict(os_cpu_count=cpus, sched_affinity=allowed, omp_num_threads=omp)
 
 
 def cpu_baseline(n, e, pos, args):
     """The oracle's CPU layout (Gephi BH, FP64, OpenMP) on the same graph:
     one warm-up iteration, then args.cpu_iters timed ones."""
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads, host = host_threads()
     kw = dict(scaling=2.0, gravity=1.0, theta=1.2, linlog=True, outbound=True,
               barnes_hut=True, threads=threads)
     p = pos.copy()
@@ -95,7 +96,7 @@ def cpu_baseline(n, e, pos, args):
     O.large_prepare(n, e, None, True)
     prep = time.perf_counter() - t1
     return dict(value=n * args.cpu_iters / dt, unit="node-iters/s",
-                cores=threads, kind="port",
+                cores=threads, kind="port", host=host,
                 sample=(f"{args.cpu_iters} iterations (after 1 warm-up) of the "
                         f"same {n}-node graph, Gephi Barnes-Hut theta=1.2, FP64, "
                         f"{threads} OpenMP threads, {dt:.1f} s"),
@@ -195,7 +196,7 @@ def offload_leg(args):
     r = 10.0 * np.sqrt(n)
     pos = rng.uniform(-r, r, size=(n, 2))
     K = args.offload_iters
-    threads = min(16, os.cpu_count() or 1)
+    threads, _ = host_threads()
     out = {}
     for name, bh in (("exact", False), ("barnes_hut", True)):
         F.layout_forceatlas2_gpu(n, e, 2, weights=w, init=pos, barnes_hut=bh, theta=1.2,
@@ -234,22 +235,11 @@ def offload_leg(args):
 
 def batched_leg(args):
     """512 SBM graphs, 200 iterations: one batch against one at a time."""
-    from forceatlas2_rc_amd import Batch, graphs
-    from forceatlas2_rc_amd.api import prepare_large
-    rng = np.random.default_rng(args.seed + 4)
-    gs, n_tot, m_tot = [], 0, 0
+    from forceatlas2_rc_amd import Batch, workloads
     t0 = time.time()
-    for g in range(args.batch_graphs):
-        n = int(round(np.exp(rng.uniform(np.log(200), np.log(20000)))))
-        k = int(rng.integers(2, 21))
-        n, e = graphs.sbm(n, k, deg_in=8.0, deg_out=1.0, seed=args.seed + 1000 + g)
-        r = 10.0 * np.sqrt(n)
-        pos = rng.uniform(-r, r, size=(n, 2))
-        mass, off, node, coef = prepare_large(n, e[:, 0], e[:, 1], np.ones(len(e)), False)
-        gs.append(dict(n=n, frm=e[:, 0], to=e[:, 1], mass=mass, inc_off=off, inc_node=node,
-                       inc_coef=coef, init=pos, theta=1.2, iterations=args.batch_iters))
-        n_tot += n
-        m_tot += len(e)
+    gs = workloads.sbm_batch(args.batch_graphs, args.batch_iters, args.seed)
+    n_tot = sum(g["n"] for g in gs)
+    m_tot = sum(len(g["frm"]) for g in gs)
     log(f"batched: {len(gs)} graphs, {n_tot} nodes, {m_tot} edges ({time.time() - t0:.1f} s)")
     it = args.batch_iters
     # warm-up: code objects, allocator
@@ -293,41 +283,9 @@ def batched_leg(args):
 
 
 def serving_requests(args):
-    from forceatlas2_rc_amd import graphs
-    from forceatlas2_rc_amd.api import prepare_large
-    rng = np.random.default_rng(args.seed + 5)
-    reqs = []
-    for q in range(args.serve_requests):
-        n = int(round(np.exp(rng.uniform(np.log(1000), np.log(args.serve_max_nodes)))))
-        deg = rng.uniform(4.0, 16.0)
-        if rng.random() < 0.5:
-            n, e = graphs.preferential_attachment(n, max(1, int(round(deg / 2))),
-                                                  seed=args.seed + 2000 + q)
-            kind = "pa"
-        else:
-            n, e = graphs.sbm(n, int(rng.integers(2, 21)), deg_in=deg - 1.0, deg_out=1.0,
-                              seed=args.seed + 2000 + q)
-            kind = "sbm"
-        r = 10.0 * np.sqrt(n)
-        pos = rng.uniform(-r, r, size=(n, 2))
-        g = dict(n=n, frm=e[:, 0], to=e[:, 1], init=pos, theta=1.2,
-                 iterations=int(rng.integers(100, 1001)))
-        if rng.random() < 0.5:
-            g["linlog"] = True
-        if rng.random() < 0.3:
-            g["adjust"] = True
-            g["size"] = rng.uniform(1.0, 3.0, n)
-        if rng.random() < 0.5:
-            g["fixed"] = rng.random(n) < 0.05
-        outbound = rng.random() < 0.3
-        g["dissuade_hubs"] = outbound
-        mass, off, node, coef = prepare_large(n, e[:, 0], e[:, 1], np.ones(len(e)), outbound)
-        g.update(mass=mass, inc_off=off, inc_node=node, inc_coef=coef)
-        prec = "double" if rng.random() < 0.25 else "single"
-        reqs.append((g, prec, kind))
-    gaps = rng.exponential(1.0 / args.serve_rate, len(reqs))
-    gaps[0] = 0.0
-    return reqs, np.cumsum(gaps)
+    from forceatlas2_rc_amd import workloads
+    return workloads.serving_requests(args.serve_requests, args.serve_rate,
+                                      args.serve_max_nodes, args.seed)
 
 
 def serve_stream(reqs, arrive, args):
@@ -444,9 +402,6 @@ def main():
     ap.add_argument("--serve-max-nodes", type=int, default=1_000_000)
     ap.add_argument("--serve-batches", type=int, default=4)
     ap.add_argument("--serve-chunk", type=int, default=25)
-    ap.add_argument("--copy-bytes", type=int, default=0,
-                    help="first run one device-to-device copy of this many "
-                         "bytes (a known-size reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU "
@@ -454,21 +409,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from forceatlas2_rc_amd import Batch
+    from forceatlas2_rc_amd import Batch, workloads
     from forceatlas2_rc_amd.api import prepare_large
 
-    if args.copy_bytes:
-        import torch
-        src = torch.empty(args.copy_bytes // 4, dtype=torch.float32, device="cuda")
-        dst = src.clone()
-        torch.cuda.synchronize()
-        del src, dst
-
     t0 = time.time()
-    n, e = make_graph(args.scale, args.edges, args.seed)
-    rng = np.random.default_rng(args.seed)
-    r = 10.0 * np.sqrt(n)
-    pos = rng.uniform(-r, r, size=(n, 2))
+    n, e, pos = workloads.rmat_large(args.scale, args.edges, args.seed)
     log(f"graph n={n} m={len(e)} ({time.time() - t0:.1f} s)")
     t0 = time.perf_counter()
     mass, off, node, coef = prepare_large(n, e[:, 0], e[:, 1],

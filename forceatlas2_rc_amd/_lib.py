@@ -117,6 +117,7 @@ PROTOTYPES = {
     "fa2_batch_positions": (c_int, [c_void_p, c_int, c_void_p]),
     "fa2_batch_forces": (c_int, [c_void_p, c_int, c_void_p]),
     "fa2_batch_speed": (c_int, [c_void_p, c_int, c_void_p]),
+    "fa2_batch_progress": (c_int, [c_void_p, c_void_p, c_void_p]),
     "fa2_batch_regions": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "fa2_batch_tree": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p]),

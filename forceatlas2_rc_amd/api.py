@@ -308,7 +308,17 @@ def _fill_params(p, gr, device, precision, profile, edges=None):
         edges.outbound = int(outbound)
         mass = off = node = coef = None
     elif "inc_off" in gr:
-        mass, off, node, coef = (gr["mass"], gr["inc_off"], gr["inc_node"], gr["inc_coef"])
+        mass = np.ascontiguousarray(gr["mass"], np.float64)
+        off = np.ascontiguousarray(gr["inc_off"], np.int32)
+        node = np.ascontiguousarray(gr["inc_node"], np.int32)
+        coef = np.ascontiguousarray(gr["inc_coef"], np.float64)
+        if (mass.shape != (n,) or off.shape != (n + 1,) or off[0] != 0
+                or np.any(np.diff(off) < 0) or node.shape != (int(off[-1]),)
+                or coef.shape != node.shape):
+            raise ValueError("mass, inc_off, inc_node and inc_coef must hold n, n + 1, "
+                             "inc_off[n] and inc_off[n] values, inc_off rising from 0")
+        if node.size and (node.min() < 0 or node.max() >= n):
+            raise ValueError("inc_node holds a node outside 0..n-1")
     else:
         mass, off, node, coef = prepare_large(n, frm, to, np.ones(frm.size) if w is None else w,
                                               outbound)
@@ -457,6 +467,15 @@ class Batch:
         if lib.fa2_batch_speed(self.h, g, _ptr(out)) != 0:
             raise GPUError(last_error())
         return out
+
+    def progress(self):
+        """(iterations applied, status) per graph: status 0, or 2 once the
+        graph's swing or traction totals stopped being numbers."""
+        done = np.zeros(self.G, np.int32)
+        status = np.zeros(self.G, np.int32)
+        if lib.fa2_batch_progress(self.h, _ptr(done), _ptr(status)) != 0:
+            raise GPUError(last_error())
+        return done, status
 
     def tree(self, g=0):
         n = self.n[g]

@@ -1,6 +1,8 @@
 // C entry points of the all-GPU path: fa2_full_run (the drop-in of
 // src/gpu_full.c:164-532) and the batch API of fa2_hip.h.
+#include <algorithm>
 #include <cstring>
+#include <ctime>
 #include <new>
 #include <vector>
 
@@ -49,6 +51,11 @@ static fa2_batch *open_batch(int ngraphs, const fa2_full_params *params,
         set_message("no graph to lay out");
         return nullptr;
     }
+    for (int g = 0; g < ngraphs; g++) // before the device is touched
+        if (const char *why = graph_args_error(&params[g], edges ? &edges[g] : nullptr)) {
+            set_message("%s", why);
+            return nullptr;
+        }
     const int device = params[0].device;
     hipStream_t st = device_stream(device);
     if (!st) return nullptr;
@@ -108,6 +115,12 @@ extern "C" int fa2_batch_speed(fa2_batch *b, int g, double out[4])
 {
     if (!b) return 1;
     return b->e->speed_state(g, out);
+}
+
+extern "C" int fa2_batch_progress(fa2_batch *b, int *done, int *status)
+{
+    if (!b) return 1;
+    return b->e->progress(done, status);
 }
 
 extern "C" int fa2_batch_tree(fa2_batch *b, int g, uint32_t *keys_sorted, uint32_t *order,
@@ -173,8 +186,75 @@ extern "C" void *fa2_batch_stream(fa2_batch *b) { return b ? (void *) b->e->stre
 
 extern "C" void fa2_batch_close(fa2_batch *b) { delete b; }
 
+// Wait for `ev`, asking the caller's interrupt callback every 0.5 ms.
+// Returns 1 (message set) if it asked to stop or the device failed.
+static int wait_polling(hipEvent_t ev, int (*interrupted)(void))
+{
+    for (;;) {
+        const hipError_t q = hipEventQuery(ev);
+        if (q == hipSuccess) return 0;
+        if (q != hipErrorNotReady) {
+            set_hip_error("the computation on the GPU failed", q);
+            return 1;
+        }
+        if (interrupted()) {
+            set_message("the layout was interrupted");
+            return 1;
+        }
+        struct timespec ts = {0, 500000};
+        nanosleep(&ts, nullptr);
+    }
+}
+
+// The iterations of fa2_full_run with an interrupt callback: chunks of
+// iterations enqueued one ahead of the one waited on, so that the device
+// never idles while the host polls. The engine also asks the callback as
+// the reference does, once per 10^6 node and incidence visits enqueued
+// (src/gpu_full.c:404-411).
+static int run_interruptible(Engine *e, const fa2_full_params *p)
+{
+    const double per = (double) p->n + (double) (p->inc_off ? p->inc_off[p->n] : 0);
+    int chunk = (int) std::min(256.0, std::max(2.0, 1e8 / per));
+    chunk += chunk & 1; // whole replays of the captured pair of iterations
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    for (hipEvent_t &x : ev)
+        if (hipEventCreateWithFlags(&x, hipEventDisableTiming) != hipSuccess) {
+            set_message("cannot create a HIP event");
+            for (hipEvent_t y : ev)
+                if (y) (void) hipEventDestroy(y);
+            return 1;
+        }
+    int status = 0, left = p->iterations, k = 0;
+    bool prev = false; // ev[k ^ 1] marks the end of a chunk not yet waited for
+    while (status == 0) {
+        bool cur = false;
+        if (left > 0) {
+            const int c = std::min(chunk, left);
+            left -= c;
+            if (e->run(c) != 0) {
+                status = 1;
+                break;
+            }
+            (void) hipEventRecord(ev[k], e->stream);
+            cur = true;
+        }
+        if (prev && wait_polling(ev[k ^ 1], p->interrupted)) status = 1;
+        if (!cur) break;
+        prev = true;
+        k ^= 1;
+    }
+    (void) hipStreamSynchronize(e->stream); // nothing may run when the batch closes
+    for (hipEvent_t y : ev) (void) hipEventDestroy(y);
+    if (status) return 1;
+    return e->sync();
+}
+
 // fa2_full_run: upload, iterate, download (src/gpu_full.c:164-532). The
-// adaptive speed stays on the device; the host waits once, at the end.
+// adaptive speed stays on the device; without an interrupt callback the host
+// waits once, at the end. Returns 1 with "the layout was interrupted" when
+// the callback returns non-zero, and with "the computation on the GPU
+// produced invalid numbers" when the swing or traction totals stop being
+// numbers (src/gpu_full.c:486-491).
 extern "C" int fa2_full_run(const fa2_full_params *p, double *pos, double *times)
 {
     for (int i = 0; i < FA2_FULL_NTIMES; i++) times[i] = 0.0;
@@ -187,7 +267,9 @@ extern "C" int fa2_full_run(const fa2_full_params *p, double *pos, double *times
     if (!b) return 1;
     const double start = fa2_now(); // the iterations and the download, as the reference
     int status = 1;
-    if (b->e->run(p->iterations) == 0 && b->e->sync() == 0 && b->e->positions(0, pos) == 0) {
+    const int ran = p->interrupted ? run_interruptible(b->e, p)
+                                   : (b->e->run(p->iterations) || b->e->sync());
+    if (ran == 0 && b->e->positions(0, pos) == 0) {
         status = 0;
         double t[FA2_FULL_NTIMES];
         b->e->stage_times(t);

@@ -77,6 +77,26 @@ struct Engine {
     virtual int set_tuning(const fa2_tuning *t) = 0;
 };
 
+// The arguments of one graph that every entry point checks before it
+// touches the device: its size, and its edge list (e non-null) or its
+// masses and incidence lists. Returns the message of the first problem, or
+// null.
+inline const char *graph_args_error(const fa2_full_params *p, const fa2_edges *e)
+{
+    if (!p) return "no graph to lay out";
+    if (p->n < 2 || p->n > (1 << 26)) return "the all-GPU layout handles between 2 and 67 million nodes";
+    if (e) {
+        if (e->m < 0 || (e->m > 0 && (!e->from || !e->to)))
+            return "an edge list needs m >= 0 and both endpoint arrays";
+        return nullptr;
+    }
+    if (!p->inc_off || !p->mass || p->inc_off[0] != 0 || p->inc_off[p->n] < 0 ||
+        (p->inc_off[p->n] > 0 && (!p->inc_node || !p->inc_coef)))
+        return "every graph needs its masses and incidence lists (inc_off[0] = 0, "
+               "inc_off[n] >= 0, inc_node and inc_coef when inc_off[n] > 0)";
+    return nullptr;
+}
+
 Engine *make_engine(int use_double);
 
 // Device memory one graph of n nodes and `incidences` incidence entries

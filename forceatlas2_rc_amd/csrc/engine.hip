@@ -1685,14 +1685,12 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     long long nsum = 0, isum = 0;
     for (int g = 0; g < G; g++) {
         const fa2_full_params *p = gr[g].p;
-        if (p->n < 2 || p->n > (1 << 26)) {
-            set_message("the all-GPU layout handles between 2 and 67 million nodes");
+        if ((gr[g].edges != nullptr) != from_edges) {
+            set_message("every graph of a batch needs its edge list, or none does");
             return 1;
         }
-        if ((gr[g].edges != nullptr) != from_edges ||
-            (from_edges && (gr[g].edges->m < 0 || (gr[g].edges->m > 0 &&
-                                                   (!gr[g].edges->from || !gr[g].edges->to))))) {
-            set_message("every graph of a batch needs its edge list, or none does");
+        if (const char *why = graph_args_error(p, gr[g].edges)) {
+            set_message("%s", why);
             return 1;
         }
         nsum += p->n;
@@ -2488,8 +2486,8 @@ template <typename R> int EngineImpl<R>::progress(int *done, int *status)
         return 1;
     }
     for (int g = 0; g < G; g++) {
-        done[g] = gs[g].done;
-        status[g] = gs[g].status;
+        if (done) done[g] = gs[g].done;
+        if (status) status[g] = gs[g].status;
     }
     return 0;
 }

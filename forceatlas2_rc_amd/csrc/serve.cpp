@@ -347,10 +347,12 @@ static int submit(fa2_server *s, const fa2_full_params *p, const fa2_edges *e, c
         set_message("no request");
         return -1;
     }
-    if (p->n < 2 || p->n > (1 << 26) || p->iterations < 1 ||
-        (e ? (e->m < 0 || (e->m > 0 && (!e->from || !e->to))) : (!p->inc_off || !p->mass))) {
-        set_message("a request needs 2..2^26 nodes, its edges or incidence lists, and "
-                    "iterations");
+    if (const char *why = graph_args_error(p, e)) {
+        set_message("%s", why);
+        return -1;
+    }
+    if (p->iterations < 1) {
+        set_message("a request needs at least one iteration");
         return -1;
     }
     auto j = std::make_unique<Job>();

@@ -110,6 +110,13 @@ int fa2_batch_forces(fa2_batch *b, int g, double *f);
  * after its last iteration. */
 int fa2_batch_speed(fa2_batch *b, int g, double out[4]);
 
+/* Iterations applied to each graph so far and its status (0, or 2 once its
+ * swing or traction totals stopped being numbers: it then stops moving, as
+ * the reference stops its layout, src/gpu_full.c:486-491), one value per
+ * graph each, after the work enqueued before the call. Either pointer may
+ * be NULL. */
+int fa2_batch_progress(fa2_batch *b, int *done, int *status);
+
 /* The Morton tree of graph g at its last iteration, in the numbering of the
  * reference (regions 0..n-2, leaves n-1..2n-2): sorted keys (n), sorted
  * permutation (n), children (2(n-1)), ranges (2(n-1)), parents (2n-1). Any

@@ -173,3 +173,68 @@ def test_weight_transform_identities():
     assert np.allclose(prepare_weights(4, w, 0.5), np.sqrt(w))
     assert np.array_equal(prepare_weights(4, w, 0.0), np.ones(4))
     assert np.array_equal(prepare_weights(3, [2.0, 2.0, 2.0], 1.0, True), np.ones(3))
+
+
+def test_graph_arguments_are_checked_before_the_device():
+    """fa2_batch_open and fa2_full_run refuse a graph without masses or
+    incidence lists, or whose lists are inconsistent, with a message and
+    before any device call (so this runs without a GPU)."""
+    lib = _lib.lib
+    n = 4
+    mass = np.full(n, 2.0)
+    off = np.array([0, 1, 2, 3, 4], np.int32)
+    node = np.array([1, 0, 3, 2], np.int32)
+    coef = np.ones(4)
+    xy = np.zeros(2 * n)
+    pos = (ctypes.c_void_p * 1)(xy.ctypes.data)
+    P = ctypes.POINTER
+
+    def params(**kw):
+        p = _lib.FullParams()
+        p.n = n
+        p.iterations = 1
+        arrs = dict(inc_off=off, inc_node=node, inc_coef=coef, mass=mass)
+        arrs.update(kw)
+        for k, ct in (("inc_off", ctypes.c_int), ("inc_node", ctypes.c_int),
+                      ("inc_coef", ctypes.c_double), ("mass", ctypes.c_double)):
+            a = arrs[k]
+            setattr(p, k, None if a is None else a.ctypes.data_as(P(ct)))
+        return p
+
+    for kw in (dict(inc_off=None), dict(mass=None), dict(inc_node=None), dict(inc_coef=None),
+               dict(inc_off=np.array([1, 1, 2, 3, 4], np.int32)),
+               dict(inc_off=np.array([0, 0, 0, 0, -1], np.int32))):
+        p = params(**kw)
+        assert lib.fa2_batch_open(1, ctypes.byref(p), pos) is None, kw
+        assert "incidence lists" in _lib.last_error(), kw
+        t = np.zeros(7)
+        assert lib.fa2_full_run(ctypes.byref(p), xy.ctypes.data_as(ctypes.c_void_p),
+                                t.ctypes.data_as(ctypes.c_void_p)) == 1
+        assert "incidence lists" in _lib.last_error(), kw
+    # a graph without incidences needs no lists
+    p = params(inc_node=None, inc_coef=None, inc_off=np.zeros(n + 1, np.int32))
+    e = _lib.Edges()
+    e.m = -1
+    assert lib.fa2_batch_open_edges(1, ctypes.byref(p), ctypes.byref(e), pos) is None
+    assert "edge list" in _lib.last_error()
+
+
+def test_prepared_lists_are_coerced_and_checked():
+    """Prepared mass and incidence lists of a graph dict are converted to the
+    C types, and lists of the wrong length are refused, before any device
+    call."""
+    from forceatlas2_rc_amd.api import _fill_params, prepare_large
+    n = 5
+    e = np.array([[0, 1], [1, 2], [3, 4], [0, 4]])
+    mass, off, node, coef = prepare_large(n, e[:, 0], e[:, 1], np.ones(4), False)
+    g = dict(n=n, frm=e[:, 0], to=e[:, 1], mass=mass.astype(np.float32), inc_off=off.astype(np.int64),
+             inc_node=node.astype(np.int64), inc_coef=coef.astype(np.float32), init=np.zeros((n, 2)))
+    p = _lib.FullParams()
+    keep, _ = _fill_params(p, g, 0, "single", False)
+    assert keep[1].dtype == np.int32 and keep[3].dtype == np.float64
+    assert [p.inc_off[i] for i in range(n + 1)] == off.tolist()
+    assert [p.inc_coef[i] for i in range(len(coef))] == coef.tolist()
+    with pytest.raises(ValueError, match="inc_off"):
+        _fill_params(p, dict(g, inc_node=node[:-1]), 0, "single", False)
+    with pytest.raises(ValueError, match="outside"):
+        _fill_params(p, dict(g, inc_node=node + n), 0, "single", False)

@@ -71,3 +71,50 @@ def test_c1_islands_offload_double():
     got = F.layout_forceatlas2_gpu(n, f["edges"], iterations=10, init=f["init"],
                                    scaling_ratio=2.0, barnes_hut=False, precision="double")
     assert np.abs(got - f["pos10"]).mean() / np.abs(f["pos10"]).mean() < 1e-9
+
+
+def edge_length_ratio(xy, edges):
+    """Mean edge length over the layout's standard deviation: the scale-free
+    measure of a final layout that the long runs are compared by."""
+    d = np.hypot(*(xy[edges[:, 0]] - xy[edges[:, 1]]).T)
+    return d.mean() / np.std(xy, ddof=1)
+
+
+@pytest.mark.parametrize("path", ["offload_single", "offload_double", "large_single",
+                                  "large_double"])
+def test_c1_islands_1000_iterations_quality(path):
+    """The full 1000-iteration layout of the islands graph (5 x 30 nodes,
+    exact repulsion, scaling 2, gravity 1), which diverges chaotically from
+    the CPU's after tens of iterations: its edge-length ratio must be within
+    2% of the golden CPU layout's (FP64, exact). The large path uses
+    Barnes-Hut theta = 1.2 as the reference's forceatlas2_large does; the
+    offload uses the exact all-pairs kernel."""
+    f = load("c1_islands.npz")
+    n, e = len(f["init"]), f["edges"]
+    prec = "double" if path.endswith("double") else "single"
+    if path.startswith("offload"):
+        got = F.layout_forceatlas2_gpu(n, e, iterations=1000, init=f["init"],
+                                       scaling_ratio=2.0, barnes_hut=False, precision=prec)
+    else:
+        got = F.layout_forceatlas2_large(n, e, iterations=1000, init=f["init"],
+                                         scaling_ratio=2.0, precision=prec)
+    assert np.all(np.isfinite(got))
+    want = edge_length_ratio(f["pos1000"], e)
+    assert edge_length_ratio(got, e) == pytest.approx(want, rel=0.02)
+
+
+def test_rmat_300_iterations_quality():
+    """R-MAT scale 14 (120,000 edges), linlog and dissuade_hubs, 300
+    iterations on the all-GPU path against the CPU layout (Gephi Barnes-Hut,
+    theta 1.2, FP64): edge-length ratio within 2%, spread within 5%."""
+    from forceatlas2_rc_amd import graphs
+    from oracle import oracle as O
+    n, e = graphs.rmat(14, 120_000, seed=41)
+    r = 10 * np.sqrt(n)
+    pos = graphs.uniform_positions(n, -r, r, 42)
+    got = F.layout_forceatlas2_large(n, e, 300, init=pos, linlog=True, dissuade_hubs=True)
+    cpu = O.layout(n, e, pos, 300, linlog=True, outbound=True, barnes_hut=True, theta=1.2,
+                   threads=8)
+    assert np.all(np.isfinite(got))
+    assert edge_length_ratio(got, e) == pytest.approx(edge_length_ratio(cpu, e), rel=0.02)
+    assert np.std(got, ddof=1) == pytest.approx(np.std(cpu, ddof=1), rel=0.05)
