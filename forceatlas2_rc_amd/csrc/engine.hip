@@ -678,7 +678,7 @@ template <typename R> __device__ inline typename Vec<R>::v4 region_record(const 
     const R w = r.x1 - r.x0, h = r.y1 - r.y0;
     const R cx = min(max(r.mx / r.m, r.x0), r.x1);
     const R cy = min(max(r.my / r.m, r.y0), r.y1);
-    return mk4<R>(cx, cy, r.m, (R) 2 * (w * w + h * h));
+    return mk4<R>(cx, cy, r.m, (R) 2 * sumsq_rn(w, h));
 }
 
 // Leaf to root, the second child to arrive at a region merging it. The
@@ -1065,7 +1065,7 @@ k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restric
             asm volatile("" ::"s"(rc.x), "s"(rc.y), "s"(rc.z), "s"(rc.w), "s"(rl.x), "s"(rl.y),
                          "s"(rl.z), "s"(rl.w));
             const R xd = me.x - rc.x, yd = me.y - rc.y;
-            const R d2 = xd * xd + yd * yd;
+            const R d2 = sumsq_rn(xd, yd);
             const uint64_t am = __builtin_amdgcn_ballot_w64(rl.z > done);
             const uint64_t fm = __builtin_amdgcn_ballot_w64(d2 * theta2 > rc.w);
             if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {

@@ -55,6 +55,17 @@ __device__ inline double frcp_r(double x)
 __device__ inline float fsqrt_r(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ inline double fsqrt_r(double x) { return __builtin_sqrt(x); }
 
+// a * a + b * b with each operation rounded on its own (no fused
+// multiply-add): the squared distance of the Barnes-Hut acceptance test and
+// the squared diagonal of a region, as the reference computes them
+// (pipeline.cl:38 turns contraction off). With the same region records, the
+// device then takes exactly the reference's decisions.
+template <typename R> __device__ inline R sumsq_rn(R a, R b)
+{
+#pragma clang fp contract(off)
+    return a * a + b * b;
+}
+
 __device__ inline float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ inline double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
 

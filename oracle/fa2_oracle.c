@@ -24,9 +24,13 @@
  *       _f  float everywhere        (the reference's precision = "single")
  *       _d  double everywhere       (the reference's precision = "double",
  *           with each centre of mass kept in its bounding box)
- *       _m  codes and tree from float32 positions, everything else in FP64
- *           with FP64 position accumulators (the FP64 restatement of the
- *           precision model of the HIP path, used as its parity gate).
+ *       _m  codes, tree, summaries, region records and the Barnes-Hut
+ *           acceptance test in float from float32 positions, the force
+ *           terms, swing, speed and displacement in FP64 with FP64 position
+ *           accumulators (the FP64 restatement of the precision model of the
+ *           HIP single-precision path, used as its parity gate: both take
+ *           the same regions as monopoles, so they differ by the rounding of
+ *           the terms only).
  *
  * Compiled with -O2 -ffp-contract=off and without fast-math, so that every
  * operation is an IEEE-754 round-to-nearest operation as written.
@@ -940,33 +944,39 @@ static void block_plan(int n, int *B_out, int *S_out)
 /* The three instantiations of the pipeline. TR: arithmetic of the codes;
  * FR: arithmetic of everything else; PR: type of the stored positions. */
 #define TR float
+#define SR float
 #define FR float
 #define PR float
 #define SFX(name) name##_f
 #include "large_body.h"
 #undef TR
+#undef SR
 #undef FR
 #undef PR
 #undef SFX
 
 #define ORC_CLAMP_CENTRE 1
 #define TR double
+#define SR double
 #define FR double
 #define PR double
 #define SFX(name) name##_d
 #include "large_body.h"
 #undef TR
+#undef SR
 #undef FR
 #undef PR
 #undef SFX
 
 #define ORC_MIXED 1
 #define TR float
+#define SR float
 #define FR double
 #define PR double
 #define SFX(name) name##_m
 #include "large_body.h"
 #undef TR
+#undef SR
 #undef FR
 #undef PR
 #undef SFX

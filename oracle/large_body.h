@@ -5,12 +5,17 @@
  * (inst/opencl/pipeline.cl, host sequence src/gpu_full.c:403-499) on the CPU.
  * Macros set by the includer:
  *   TR  arithmetic of the bounding box and the Morton codes
- *   FR  arithmetic of the tree summaries, forces, swing and displacement
+ *   SR  arithmetic of the tree summaries, the region records and the
+ *       Barnes-Hut acceptance test
+ *   FR  arithmetic of the forces, swing and displacement
  *   PR  type of the stored positions
  *   SFX(name)  suffix of the exported names
  *   ORC_MIXED  defined for the FP64 model of the HIP single-precision path:
- *              positions are FP64 accumulators, the codes and the forces are
- *              computed from their float32 roundings.
+ *              positions are FP64 accumulators; the codes, the summaries,
+ *              the region records and the acceptance test are computed in
+ *              float32 from their float32 roundings, as the device does, so
+ *              that both approximate the same regions; the force terms are
+ *              FP64.
  *   ORC_CLAMP_CENTRE  keep each centre of mass inside its bounding box (a
  *              no-op in exact arithmetic; the reference's rounded centre can
  *              leave the box of a cluster of coincident nodes).
@@ -61,7 +66,7 @@ void SFX(orc_large_tree)(int n, const PR *x, const PR *y, uint32_t *key,
 
 /* Bottom-up summaries (pipeline.cl:276-305): nd[8c] = (min x, min y, max x,
  * max y, mass, sum mx, sum my) of node c; a region merges left then right. */
-static void SFX(agg_node)(int n, int r, const int *child, FR *nd)
+static void SFX(agg_node)(int n, int r, const int *child, SR *nd)
 {
     /* post-order without recursion: the tree can be 64 levels deep */
     int stack[200];
@@ -72,11 +77,11 @@ static void SFX(agg_node)(int n, int r, const int *child, FR *nd)
         const int a = child[2 * c], b = child[2 * c + 1];
         int pushed = 0;
         /* nd[8*x + 7] marks a completed region (leaves are always ready) */
-        if (a < n - 1 && nd[8 * a + 7] == (FR) 0) {
+        if (a < n - 1 && nd[8 * a + 7] == (SR) 0) {
             stack[sp++] = a;
             pushed = 1;
         }
-        if (b < n - 1 && nd[8 * b + 7] == (FR) 0) {
+        if (b < n - 1 && nd[8 * b + 7] == (SR) 0) {
             stack[sp++] = b;
             pushed = 1;
         }
@@ -89,7 +94,7 @@ static void SFX(agg_node)(int n, int r, const int *child, FR *nd)
         nd[8 * c + 4] = nd[8 * a + 4] + nd[8 * b + 4];
         nd[8 * c + 5] = nd[8 * a + 5] + nd[8 * b + 5];
         nd[8 * c + 6] = nd[8 * a + 6] + nd[8 * b + 6];
-        nd[8 * c + 7] = (FR) 1;
+        nd[8 * c + 7] = (SR) 1;
     }
 }
 
@@ -107,14 +112,14 @@ int SFX(orc_large_regions)(int n, const PR *x, const PR *y, const double *mass,
     int *child = (int *) malloc(2 * (nn - 1) * sizeof(int));
     int *range = (int *) malloc(2 * (nn - 1) * sizeof(int));
     int *parent = (int *) malloc((2 * nn - 1) * sizeof(int));
-    FR *nd = (FR *) malloc(8 * (2 * nn - 1) * sizeof(FR));
+    SR *nd = (SR *) malloc(8 * (2 * nn - 1) * sizeof(SR));
     int status = 1;
     if (!key || !ks || !order || !child || !range || !parent || !nd) goto out;
     SFX(orc_large_tree)(n, x, y, key, ks, order, child, range, parent);
     for (int k = 0; k < n; k++) {
         const uint32_t i = order[k];
-        FR *L = nd + 8 * (nn - 1 + (size_t) k);
-        const FR px = POS_FR(x[i]), py = POS_FR(y[i]), pm = (FR) mass[i];
+        SR *L = nd + 8 * (nn - 1 + (size_t) k);
+        const SR px = (SR) POS_FR(x[i]), py = (SR) POS_FR(y[i]), pm = (SR) mass[i];
         L[0] = px;
         L[1] = py;
         L[2] = px;
@@ -122,14 +127,14 @@ int SFX(orc_large_regions)(int n, const PR *x, const PR *y, const double *mass,
         L[4] = pm;
         L[5] = px * pm;
         L[6] = py * pm;
-        L[7] = (FR) 1;
+        L[7] = (SR) 1;
     }
-    for (int r = 0; r < n - 1; r++) nd[8 * r + 7] = (FR) 0;
+    for (int r = 0; r < n - 1; r++) nd[8 * r + 7] = (SR) 0;
     SFX(agg_node)(n, 0, child, nd);
     for (int r = 0; r < n - 1; r++) {
-        const FR m = nd[8 * r + 4];
-        const FR w = nd[8 * r + 2] - nd[8 * r];
-        const FR h = nd[8 * r + 3] - nd[8 * r + 1];
+        const SR m = nd[8 * r + 4];
+        const SR w = nd[8 * r + 2] - nd[8 * r];
+        const SR h = nd[8 * r + 3] - nd[8 * r + 1];
 #ifdef ORC_CLAMP_CENTRE
         ni_out[4 * r] = (double) ORC_MIN(ORC_MAX(nd[8 * r + 5] / m, nd[8 * r]), nd[8 * r + 2]);
         ni_out[4 * r + 1] = (double) ORC_MIN(ORC_MAX(nd[8 * r + 6] / m, nd[8 * r + 1]), nd[8 * r + 3]);
@@ -138,7 +143,7 @@ int SFX(orc_large_regions)(int n, const PR *x, const PR *y, const double *mass,
         ni_out[4 * r + 1] = (double) (nd[8 * r + 6] / m);
 #endif
         ni_out[4 * r + 2] = (double) m;
-        ni_out[4 * r + 3] = (double) ((FR) 2 * (w * w + h * h));
+        ni_out[4 * r + 3] = (double) ((SR) 2 * (w * w + h * h));
     }
     status = 0;
 out:
@@ -198,8 +203,8 @@ int SFX(orc_large_run)(const orc_large_params *p, double *pos, int iterations,
     FR *ms = (FR *) malloc(nn * sizeof(FR));
     FR *sz = (FR *) malloc(nn * sizeof(FR));
     FR *psort = (FR *) malloc(4 * nn * sizeof(FR));
-    FR *nd = (FR *) malloc(8 * (2 * nn - 1) * sizeof(FR));
-    FR *ni = (FR *) malloc(4 * (nn - 1) * sizeof(FR));
+    SR *nd = (SR *) malloc(8 * (2 * nn - 1) * sizeof(SR));
+    SR *ni = (SR *) malloc(4 * (nn - 1) * sizeof(SR));
     FR *part = (FR *) malloc(2 * (size_t) B * sizeof(FR));
     uint32_t *key = (uint32_t *) malloc(nn * sizeof(uint32_t));
     uint32_t *ks = (uint32_t *) malloc(nn * sizeof(uint32_t));
@@ -219,7 +224,7 @@ int SFX(orc_large_run)(const orc_large_params *p, double *pos, int iterations,
         sz[i] = p->adjust ? (FR) p->size[i] : (FR) 0;
     }
     const FR scaling = (FR) p->scaling;
-    const FR theta2 = (FR) (p->theta * p->theta);
+    const SR theta2 = (SR) (p->theta * p->theta);
     const FR gravity = (FR) p->gravity;
     double speed = 1.0, eff = 1.0;
 
@@ -236,9 +241,9 @@ int SFX(orc_large_run)(const orc_large_params *p, double *pos, int iterations,
             psort[4 * k + 3] = sz[i];
         }
         for (int k = 0; k < n; k++) {
-            FR *L = nd + 8 * (nn - 1 + (size_t) k);
-            const FR px = psort[4 * k], py = psort[4 * k + 1],
-                     pm = psort[4 * k + 2];
+            SR *L = nd + 8 * (nn - 1 + (size_t) k);
+            const SR px = (SR) psort[4 * k], py = (SR) psort[4 * k + 1],
+                     pm = (SR) psort[4 * k + 2];
             L[0] = px;
             L[1] = py;
             L[2] = px;
@@ -246,14 +251,14 @@ int SFX(orc_large_run)(const orc_large_params *p, double *pos, int iterations,
             L[4] = pm;
             L[5] = px * pm;
             L[6] = py * pm;
-            L[7] = (FR) 1;
+            L[7] = (SR) 1;
         }
-        for (int r = 0; r < n - 1; r++) nd[8 * r + 7] = (FR) 0;
+        for (int r = 0; r < n - 1; r++) nd[8 * r + 7] = (SR) 0;
         SFX(agg_node)(n, 0, child, nd);
         for (int r = 0; r < n - 1; r++) { /* pipeline.cl:312-323 */
-            const FR m = nd[8 * r + 4];
-            const FR w = nd[8 * r + 2] - nd[8 * r];
-            const FR h = nd[8 * r + 3] - nd[8 * r + 1];
+            const SR m = nd[8 * r + 4];
+            const SR w = nd[8 * r + 2] - nd[8 * r];
+            const SR h = nd[8 * r + 3] - nd[8 * r + 1];
 #ifdef ORC_CLAMP_CENTRE
             /* the centre of mass stays in the bounding box of the region, as
              * in exact arithmetic (the HIP path does the same) */
@@ -264,7 +269,7 @@ int SFX(orc_large_run)(const orc_large_params *p, double *pos, int iterations,
             ni[4 * r + 1] = nd[8 * r + 6] / m;
 #endif
             ni[4 * r + 2] = m;
-            ni[4 * r + 3] = (FR) 2 * (w * w + h * h);
+            ni[4 * r + 3] = (SR) 2 * (w * w + h * h);
         }
 
         /* 5: repulsion (pipeline.cl:332-383) */
@@ -281,10 +286,13 @@ int SFX(orc_large_run)(const orc_large_params *p, double *pos, int iterations,
                 if (r >= n - 1) {
                     first = last = r - (n - 1);
                 } else {
-                    const FR xd = xi - ni[4 * r], yd = yi - ni[4 * r + 1];
-                    const FR d2 = xd * xd + yd * yd;
-                    if (d2 * theta2 > ni[4 * r + 3]) {
-                        const FR g = scaling * (mi * ni[4 * r + 2]) / d2;
+                    /* the acceptance test in the summaries' arithmetic, the
+                     * term in the forces' (the same when SR is FR) */
+                    const SR txd = (SR) xi - ni[4 * r], tyd = (SR) yi - ni[4 * r + 1];
+                    if ((txd * txd + tyd * tyd) * theta2 > ni[4 * r + 3]) {
+                        const FR xd = xi - (FR) ni[4 * r], yd = yi - (FR) ni[4 * r + 1];
+                        const FR d2 = xd * xd + yd * yd;
+                        const FR g = scaling * (mi * (FR) ni[4 * r + 2]) / d2;
                         ax += xd * g;
                         ay += yd * g;
                         continue;

@@ -113,8 +113,8 @@ def dfs_skip(child, n):
                                               ("double", 1024, "d"), ("double", 70000, "d")])
 def test_region_records(precision, n, mode):
     """Every region's traversal record (pipeline.cl:276-323) against the
-    oracle on the same positions: masses and centres bit-exact, size^2
-    within one rounding (the device may fuse w * w + h * h); skip links are
+    oracle on the same positions: masses, centres and size^2 bit-exact
+    (w * w + h * h is rounded term by term, as the reference does); skip links are
     the depth-first successors in the oracle's tree. The blocks of the first
     summary level hold 1024 nodes in single precision and 512 in double, and
     the units above them 16 and 256 blocks: 1024-9000 nodes need the second
@@ -132,8 +132,7 @@ def test_region_records(precision, n, mode):
     ref = O.large_regions(pos[:, 0], pos[:, 1], mass, mode)
     assert np.array_equal(rec[:, 2], ref[:, 2])
     assert np.array_equal(rec[:, :2], ref[:, :2])
-    eps = np.finfo(np.float32 if precision == "single" else np.float64).eps
-    assert np.all(np.abs(rec[:, 3] - ref[:, 3]) <= 2 * eps * ref[:, 3])
+    assert np.array_equal(rec[:, 3], ref[:, 3])
     tree = O.large_tree(pos[:, 0], pos[:, 1], mode)
     assert np.array_equal(skip, dfs_skip(tree["child"], n))
 
