@@ -107,6 +107,9 @@ PROTOTYPES = {
     "fa2_tree_build": (None, [POINTER(Tree), c_int, c_void_p, c_void_p,
                               c_void_p, c_void_p, c_int]),
     "fa2_gpu_kernel_time": (c_int, [c_void_p, POINTER(c_double)]),
+    "fa2_gpu_repulse_exact_noalloc": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                                              c_void_p, c_void_p, c_double, c_void_p,
+                                              c_void_p, c_void_p]),
     "fa2_batch_open": (c_void_p, [c_int, POINTER(FullParams),
                                   POINTER(c_void_p)]),
     "fa2_batch_open_edges": (c_void_p, [c_int, POINTER(FullParams), POINTER(Edges),

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "common.h"
 #include "kernels_common.h"
@@ -121,12 +122,16 @@ k_exact(int n, int slice_len, const typename Vec<R>::v4 *__restrict__ p,
     }
 }
 
-// f[i] = sum over slices, in slice order, in FP64.
-template <typename R>
+// f[i] = sum over slices, in slice order, in FP64. With ADD the force is
+// added into dx[i], dy[i] instead, rounded as the host adds it to its
+// arrays (a product, then a sum: no fused multiply-add).
+template <typename R, bool ADD = false>
 __global__ void k_sum_slices(int n, int slices, const typename Vec<R>::v2 *__restrict__ partial,
                              const double *__restrict__ mass, double scaling,
-                             double2 *__restrict__ f)
+                             double2 *__restrict__ f, double *__restrict__ dx = nullptr,
+                             double *__restrict__ dy = nullptr)
 {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double sx = 0.0, sy = 0.0;
@@ -136,7 +141,38 @@ __global__ void k_sum_slices(int n, int slices, const typename Vec<R>::v2 *__res
         sy += (double) v.y;
     }
     const double sm = scaling * mass[i]; // the factor scaling * m_i of every term
-    f[i] = make_double2(sx * sm, sy * sm);
+    if (ADD) {
+        dx[i] += sx * sm;
+        dy[i] += sy * sm;
+    } else {
+        f[i] = make_double2(sx * sm, sy * sm);
+    }
+}
+
+// dx[i] += f[i].x, dy[i] += f[i].y.
+__global__ void k_add_forces(int n, const double2 *__restrict__ f, double *__restrict__ dx,
+                             double *__restrict__ dy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 v = f[i];
+    dx[i] += v.x;
+    dy[i] += v.y;
+}
+
+// Work split of the exact repulsion of n nodes: target blocks of 1024 nodes
+// x source slices, at least about 2048 blocks in all, slices a whole number
+// of LDS tiles.
+inline void exact_plan(int n, int *slices_out, int *slice_len_out)
+{
+    const size_t nn = (size_t) n;
+    const int tblocks = (int) ceil_div(nn, kExactThreads * kExactTargets);
+    int slices = (int) ceil_div(2048, tblocks);
+    if (slices > 64) slices = 64;
+    int slice_len = (int) ceil_div(nn, slices);
+    slice_len = (int) (ceil_div(slice_len, kExactTile) * kExactTile);
+    *slices_out = (int) ceil_div(nn, slice_len);
+    *slice_len_out = slice_len;
 }
 
 // Regions of the host tree: rr = (cx, cy, mass, size^2), ri = (start, count,
@@ -292,13 +328,8 @@ extern "C" fa2_gpu *fa2_gpu_open(int device, int use_double, int n, int adjust,
     const size_t nn = (size_t) n;
     const size_t rs = g->use_double ? sizeof(double) : sizeof(float);
 
-    // Exact: target blocks x source slices, at least ~2048 blocks in all.
-    const int tblocks = (int) ceil_div(nn, kExactThreads * kExactTargets);
-    int slices = (int) ceil_div(2048, tblocks);
-    if (slices > 64) slices = 64;
-    int slice_len = (int) ceil_div(nn, slices);
-    slice_len = (int) (ceil_div(slice_len, kExactTile) * kExactTile);
-    slices = (int) ceil_div(nn, slice_len);
+    int slices, slice_len;
+    exact_plan(n, &slices, &slice_len);
     g->slices = slices;
     g->slice_len = slice_len;
 
@@ -506,5 +537,114 @@ extern "C" int fa2_gpu_repulse_bh(fa2_gpu *g, const fa2_tree *t, double scaling,
         }
     }
     g->t_fetch += fa2_now() - t0;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The exact repulsion without allocation: fa2_gpu_repulse_exact_noalloc
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// Largest node count, and the bytes of scratch it needs: in single
+// precision the packed nodes (16 bytes each) and the per-slice partial
+// forces (8 bytes per node and slice, about 2048 x 1024 entries at most);
+// in double precision the FP64 forces (16 bytes per node). Up to 2^18 nodes
+// this is at most 21.9 MiB (261,120 nodes in 9 slices).
+constexpr int kNoAllocMaxNodes = 1 << 18;
+constexpr size_t kScratchBytes = (size_t) 24 << 20;
+constexpr int kScratchDevices = 64;
+
+// The scratch of every call on a device. A file-scope device array: each
+// GPU holds its own copy, from the first call on it to the end of the
+// process. In host code `scratch` only names the symbol: kernels get the
+// device address that hipGetSymbolAddress gives for the current device.
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kScratchDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current: looked
+// up once per device, then kept.
+unsigned char *scratch_address(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_hip_error("could not find the scratch memory of the device", e);
+            return nullptr;
+        }
+        g_scratch_ptr[device] = (unsigned char *) ptr;
+    }
+    return g_scratch_ptr[device];
+}
+
+inline size_t round256(size_t b) { return (b + 255) & ~(size_t) 255; }
+
+} // namespace
+
+extern "C" int fa2_gpu_repulse_exact_noalloc(int device, int use_double, int n, int adjust,
+                                             const double *x, const double *y,
+                                             const double *mass, const double *size,
+                                             double scaling, double *dx, double *dy,
+                                             void *stream)
+{
+    if (device < 0 || device >= device_count() || device >= kScratchDevices) {
+        set_message("there is no GPU device %d", device + 1);
+        return 1;
+    }
+    if (n < 1 || n > kNoAllocMaxNodes) {
+        set_message("the exact repulsion without allocation handles 1 to %d nodes (got %d)",
+                    kNoAllocMaxNodes, n);
+        return 1;
+    }
+    if (!x || !y || !mass || !dx || !dy || (adjust && !size)) {
+        set_message("missing device arrays (x, y, mass, dx, dy, and size with prevent_overlap)");
+        return 1;
+    }
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        set_hip_error("could not select the GPU device", e);
+        return 1;
+    }
+    unsigned char *base = scratch_address(device);
+    if (!base) return 1;
+    const hipStream_t st = (hipStream_t) stream;
+    const size_t nn = (size_t) n;
+    const unsigned nb = (unsigned) ceil_div(nn, 256);
+    if (use_double) { // the reference's operations, in order, as fa2_gpu_repulse_exact
+        double2 *f = (double2 *) base;
+        e = launch_exact_f64(st, n, scaling, adjust != 0, x, y, mass, adjust ? size : nullptr, f);
+        if (e == hipSuccess) {
+            k_add_forces<<<nb, 256, 0, st>>>(n, f, dx, dy);
+            e = hipGetLastError();
+        }
+    } else {
+        int slices, slice_len;
+        exact_plan(n, &slices, &slice_len);
+        const size_t p_bytes = round256(nn * sizeof(float4));
+        if (p_bytes + (size_t) slices * nn * sizeof(float2) > kScratchBytes) {
+            set_message("the exact repulsion of %d nodes needs more scratch than the %zu bytes "
+                        "held for it", n, kScratchBytes);
+            return 1;
+        }
+        float4 *p = (float4 *) base;
+        float2 *partial = (float2 *) (base + p_bytes);
+        k_pack_nodes<float><<<nb, 256, 0, st>>>(n, x, y, mass, adjust ? size : nullptr, p);
+        dim3 grid(ceil_div(n, kExactThreads * kExactTargets), slices);
+        if (adjust)
+            k_exact<float, true><<<grid, kExactThreads, 0, st>>>(n, slice_len, p, partial);
+        else
+            k_exact<float, false><<<grid, kExactThreads, 0, st>>>(n, slice_len, p, partial);
+        k_sum_slices<float, true><<<nb, 256, 0, st>>>(n, slices, partial, mass, scaling,
+                                                      nullptr, dx, dy);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+        set_hip_error("the computation on the GPU failed", e);
+        return 1;
+    }
     return 0;
 }

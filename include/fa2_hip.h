@@ -60,6 +60,28 @@ void fa2_tree_build(fa2_tree *t, int n, const double *x, const double *y, const 
  * returns host wall times. */
 int fa2_gpu_kernel_time(const fa2_gpu *g, double *ms);
 
+/* The exact all-pairs repulsion of fa2_gpu_repulse_exact without any device
+ * allocation. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). There is no session: x, y, mass (n
+ * each), size (n, read only with adjust, and then required) and dx, dy (n
+ * each) are DEVICE arrays of doubles on `device`, and the force is added
+ * into dx, dy as fa2_gpu_repulse_exact adds it into its host arrays, with
+ * the same kernels and the same result bit for bit. The work is enqueued on
+ * `stream` (a hipStream_t; NULL for the null stream) and the call returns
+ * without waiting: synchronise the stream before reading dx, dy. Partial
+ * results live in a scratch array of 24 MiB that the library holds on each
+ * device it runs on, from the first call there to the end of the process;
+ * n is limited to 1..2^18 nodes (262,144). Every call on a device shares
+ * that device's scratch: calls that overlap on one device (on different
+ * streams, or from several host threads) race and give wrong results;
+ * calls in order on one stream are safe. 0 on success, 1 on error (message
+ * in fa2_gpu_error). */
+int fa2_gpu_repulse_exact_noalloc(int device, int use_double, int n, int adjust,
+                                  const double *x, const double *y, const double *mass,
+                                  const double *size, double scaling, double *dx, double *dy,
+                                  void *stream);
+
 /* ----- batches ---------------------------------------------------------- */
 
 typedef struct fa2_batch fa2_batch;
