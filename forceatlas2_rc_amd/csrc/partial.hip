@@ -15,6 +15,7 @@
 //    repeat the reference's CPU operations in order.
 //  * Barnes-Hut: one lane per node in tree order, stackless traversal along
 //    the skip links of the host tree, regions packed as two 16-byte records.
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -193,52 +194,139 @@ __global__ void k_pack_regions(int nreg, const fa2_region *__restrict__ reg,
     ri[r] = make_int4(g.start, g.count, g.nchild > 0 ? g.child + g.nchild - 1 : -1, g.skip);
 }
 
-// Barnes-Hut over the host tree (reference kernel fa2_bh, src/gpu.c:82-123):
-// the same regions in the same order as the CPU stack traversal.
-template <typename R, bool ADJUST>
-__global__ void __launch_bounds__(kBhThreads)
-k_bh_host_tree(int n, R scaling, R theta2, const typename Vec<R>::v4 *__restrict__ p,
-               const typename Vec<R>::v4 *__restrict__ rr, const int4 *__restrict__ ri,
-               typename Vec<R>::v2 *__restrict__ f)
+// One region of the host tree as the walk reads it: (cx, cy, mass, size^2)
+// and (first node, node count, last child or -1, skip or -1), 32 bytes, one
+// scalar load.
+struct alignas(32) HostRec {
+    float4 c;
+    int4 l;
+};
+
+// Region records of the single-precision walk, rounded to float32 as the
+// reference's converter does (src/gpu.c:501-519: cx, cy, mass and size^2
+// each rounded once).
+__global__ void k_pack_host_records(int nreg, const fa2_region *__restrict__ reg,
+                                    HostRec *__restrict__ rec)
 {
-    using V4 = typename Vec<R>::v4;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const V4 me = p[k];
-    const R smi = scaling * me.z;
-    R ax = (R) 0, ay = (R) 0;
-    int cur = 0;
-    while (cur >= 0) {
-        const int4 info = ri[cur];
-        if (info.y > 1) {
-            const V4 c = rr[cur];
-            const R xd = me.x - c.x, yd = me.y - c.y;
-            const R d2 = xd * xd + yd * yd;
-            if (d2 * theta2 > c.w) {
-                const R g = (smi * c.z) * frcp_r(d2);
-                ax += xd * g;
-                ay += yd * g;
-                cur = info.w;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nreg) return;
+    const fa2_region g = reg[r];
+    HostRec h;
+    h.c = make_float4((float) g.cx, (float) g.cy, (float) g.mass, (float) (g.size * g.size));
+    h.l = make_int4(g.start, g.count, g.nchild > 0 ? g.child + g.nchild - 1 : -1, g.skip);
+    rec[r] = h;
+}
+
+// Barnes-Hut over the host Gephi tree, wave-synchronous. The 64 lanes of a
+// wave are 64 consecutive nodes of the tree order, and the wave walks the
+// union of their traversals (the reference's fa2_bh, src/gpu.c:82-123, walks
+// one node per work item): one region per step, read with one uniform
+// scalar load, tested by every lane at once.
+//
+// A lane takes part in a region only where its own traversal goes. The
+// traversal visits a region's children from the last to the first, and the
+// children split the region's nodes in that order of positions, so what a
+// lane has already covered (regions it accepted or whose nodes it visited)
+// is always the positions from `low` up to the end of the tree order: a
+// region [s, e] lies on the lane's own path exactly when e < low. Each lane
+// therefore meets its regions in the reference's order and sums the same
+// terms in that order.
+//
+// The acceptance test d^2 theta^2 > size^2 is taken in float32 on the
+// float32 records with each operation rounded on its own, as the
+// reference's single-precision kernel takes it; each term enters the lane's
+// FP64 sum through exact FP64 multiply-adds, and the factor scaling * m_i is
+// applied once at the end. With node sizes, a pair's term is computed in
+// FP64 from the float32 inputs (the force jumps as two discs touch).
+// `p` holds kHostLeafLoad records of padding after the last node: the nodes
+// of a leaf region are loaded together. The forces leave in FP64.
+constexpr int kHostLeafLoad = 8;
+
+template <bool ADJUST>
+__global__ void __launch_bounds__(kBhThreads)
+k_bh_walk(int n, int nreg, float scaling, float theta2, const float4 *__restrict__ p,
+          const HostRec *__restrict__ rec, double2 *__restrict__ f)
+{
+    const int k0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = k0 < n ? k0 : n - 1;
+    const bool valid = k0 < n;
+    const float4 me = p[k];
+    double sx = 0.0, sy = 0.0;
+    int low = valid ? INT_MAX : INT_MIN; // lanes past the end never take part
+    int c = 0;
+    // a valid tree visits each region at most once; the bound ends the walk
+    // of a malformed one
+    for (int steps = 0; c >= 0 && c < nreg && steps < nreg; steps++) {
+        c = __builtin_amdgcn_readfirstlane(c);
+        const HostRec h = rec[c];
+        const int s = h.l.x, cnt = h.l.y;
+        const uint64_t am = __builtin_amdgcn_ballot_w64(s + cnt - 1 < low);
+        if (am == 0) { // no lane walks this region
+            c = h.l.w;
+            continue;
+        }
+        uint64_t tm = am;
+        if (cnt > 1) {
+            const float xd = me.x - h.c.x, yd = me.y - h.c.y;
+            const float d2 = sumsq_rn(xd, yd);
+            const uint64_t fm = __builtin_amdgcn_ballot_w64(d2 * theta2 > h.c.w);
+            if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {
+                const double w = (double) (h.c.z * frcp_r(d2));
+                sx = __builtin_fma((double) xd, w, sx);
+                sy = __builtin_fma((double) yd, w, sy);
+                low = s;
+            }
+            tm = am & ~fm;
+            if (tm == 0) {
+                c = h.l.w;
                 continue;
             }
-            if (info.z >= 0) {
-                cur = info.z;
+            if (h.l.z >= 0) { // some lanes open it: its last child next
+                c = h.l.z;
                 continue;
             }
         }
-        for (int j = info.x; j < info.x + info.y; j++) {
-            if (j == k) continue;
-            const V4 o = p[j];
-            const R xd = me.x - o.x, yd = me.y - o.y;
-            const R d2 = xd * xd + yd * yd;
-            const R g = ADJUST ? pair_factor_adjust<R>(xd, yd, smi * o.z, me.w, o.w)
-                               : pair_factor<R>(d2, smi * o.z);
-            ax += xd * g;
-            ay += yd * g;
+        // the region's nodes one by one (a leaf of at most 8 nodes in a tree
+        // of fa2_tree_build; longer lists in pieces of 8)
+        for (int b = s; b < s + cnt; b += kHostLeafLoad) {
+            float4 o[kHostLeafLoad];
+#pragma unroll
+            for (int u = 0; u < kHostLeafLoad; u++) o[u] = p[b + u];
+            if (__builtin_amdgcn_inverse_ballot_w64(tm)) {
+#pragma unroll
+                for (int u = 0; u < kHostLeafLoad; u++) {
+                    if (b + u >= s + cnt) break;
+                    if (ADJUST) {
+                        // the pair in FP64 (src/gpu.c:38-47), a lane with
+                        // itself adding zero (a zero offset)
+                        const double dx = (double) me.x - (double) o[u].x;
+                        const double dy = (double) me.y - (double) o[u].y;
+                        const double d = __builtin_sqrt(sumsq_rn(dx, dy)) -
+                                         ((double) me.w + (double) o[u].w);
+                        const double mo = (double) o[u].z;
+                        double g = d > 0.0 ? mo / (d * d) : (d < 0.0 ? 100.0 * mo : 0.0);
+                        g = b + u == k ? 0.0 : g;
+                        sx = __builtin_fma(dx, g, sx);
+                        sy = __builtin_fma(dy, g, sy);
+                    } else {
+                        const float xd = me.x - o[u].x, yd = me.y - o[u].y;
+                        const float d2 = xd * xd + yd * yd;
+                        // the lane itself, and coincident nodes: d2 = 0, no term
+                        float w = d2 > 0.0f ? o[u].z * frcp_r(d2) : 0.0f;
+                        asm volatile("" : "+v"(w));
+                        sx = __builtin_fma((double) xd, (double) w, sx);
+                        sy = __builtin_fma((double) yd, (double) w, sy);
+                    }
+                }
+                low = s;
+            }
         }
-        cur = info.w;
+        c = h.l.w;
     }
-    f[k] = typename Vec<R>::v2{ax, ay};
+    if (valid) {
+        const double sm = (double) scaling * (double) me.z;
+        f[k] = make_double2(sx * sm, sy * sm);
+    }
 }
 
 } // namespace
@@ -260,8 +348,9 @@ struct fa2_gpu {
     DevBuf<double2> d_f64;
     DevBuf<unsigned char> d_f; // BH forces in tree order
     DevBuf<fa2_region> d_reg;
-    DevBuf<unsigned char> d_rr;
-    DevBuf<int4> d_ri;
+    DevBuf<unsigned char> d_rr; // double sessions: region records (cx, cy, mass, size^2)
+    DevBuf<int4> d_ri;          // and (start, count, last child, skip)
+    DevBuf<HostRec> d_hrec;     // single sessions: both in one 32-byte record
     PinBuf<double> h_in;
     PinBuf<double2> h_f64;
     PinBuf<unsigned char> h_f;
@@ -335,15 +424,22 @@ extern "C" fa2_gpu *fa2_gpu_open(int device, int use_double, int n, int adjust,
 
     HipStatus hs;
     hs.check(g->d_in.alloc(4 * nn), "alloc");
-    hs.check(g->d_p.alloc(4 * nn * rs), "alloc");
+    // packed nodes, with the padding records that the Barnes-Hut walk loads
+    // after the last node (zero)
+    hs.check(g->d_p.alloc(4 * (nn + kHostLeafLoad) * rs), "alloc");
+    if (hs.ok()) hs.check(hipMemset(g->d_p.p, 0, 4 * (nn + kHostLeafLoad) * rs), "clear");
     hs.check(g->h_in.alloc(4 * nn), "alloc");
     if (g->barnes_hut) {
-        hs.check(g->d_f.alloc(2 * nn * rs), "alloc");
-        hs.check(g->h_f.alloc(2 * nn * rs), "alloc");
+        hs.check(g->d_f.alloc(2 * nn * sizeof(double)), "alloc"); // FP64 forces
+        hs.check(g->h_f.alloc(2 * nn * sizeof(double)), "alloc");
         hs.check(g->d_reg.alloc(2 * nn), "alloc");
         hs.check(g->h_reg.alloc(2 * nn), "alloc");
-        hs.check(g->d_rr.alloc(2 * nn * 4 * rs), "alloc");
-        hs.check(g->d_ri.alloc(2 * nn), "alloc");
+        if (g->use_double) {
+            hs.check(g->d_rr.alloc(2 * nn * 4 * rs), "alloc");
+            hs.check(g->d_ri.alloc(2 * nn), "alloc");
+        } else {
+            hs.check(g->d_hrec.alloc(2 * nn), "alloc");
+        }
     } else {
         if (!g->use_double) hs.check(g->d_partial.alloc((size_t) slices * nn * 2 * rs), "alloc");
         hs.check(g->d_f64.alloc(nn), "alloc");
@@ -442,14 +538,17 @@ template <typename R>
 static hipError_t launch_bh(fa2_gpu *g, int nreg, double scaling, double theta)
 {
     using V4 = typename Vec<R>::v4;
-    using V2 = typename Vec<R>::v2;
     const int n = g->n;
     const double *in = g->d_in.p;
     k_pack_nodes<R><<<ceil_div(n, 256), 256, 0, g->stream>>>(
         n, in, in + n, in + 2 * (size_t) n, g->adjust ? in + 3 * (size_t) n : nullptr,
         (V4 *) g->d_p.p);
-    k_pack_regions<R><<<ceil_div(nreg, 256), 256, 0, g->stream>>>(
-        nreg, g->d_reg.p, (V4 *) g->d_rr.p, g->d_ri.p);
+    if (std::is_same<R, double>::value)
+        k_pack_regions<R><<<ceil_div(nreg, 256), 256, 0, g->stream>>>(
+            nreg, g->d_reg.p, (V4 *) g->d_rr.p, g->d_ri.p);
+    else
+        k_pack_host_records<<<ceil_div(nreg, 256), 256, 0, g->stream>>>(nreg, g->d_reg.p,
+                                                                        g->d_hrec.p);
     if (!g->ev[0]) {
         (void) hipEventCreate(&g->ev[0]);
         (void) hipEventCreate(&g->ev[1]);
@@ -464,18 +563,33 @@ static hipError_t launch_bh(fa2_gpu *g, int nreg, double scaling, double theta)
         g->timed = 1;
         return e;
     }
-    const R th2 = (R) (theta * theta);
+    const float th2 = (float) (theta * theta);
+    const unsigned blocks = (unsigned) ceil_div(n, kBhThreads);
     if (g->adjust)
-        k_bh_host_tree<R, true><<<ceil_div(n, kBhThreads), kBhThreads, 0, g->stream>>>(
-            n, (R) scaling, th2, (const V4 *) g->d_p.p, (const V4 *) g->d_rr.p, g->d_ri.p,
-            (V2 *) g->d_f.p);
+        k_bh_walk<true><<<blocks, kBhThreads, 0, g->stream>>>(
+            n, nreg, (float) scaling, th2, (const float4 *) g->d_p.p, g->d_hrec.p,
+            (double2 *) g->d_f.p);
     else
-        k_bh_host_tree<R, false><<<ceil_div(n, kBhThreads), kBhThreads, 0, g->stream>>>(
-            n, (R) scaling, th2, (const V4 *) g->d_p.p, (const V4 *) g->d_rr.p, g->d_ri.p,
-            (V2 *) g->d_f.p);
+        k_bh_walk<false><<<blocks, kBhThreads, 0, g->stream>>>(
+            n, nreg, (float) scaling, th2, (const float4 *) g->d_p.p, g->d_hrec.p,
+            (double2 *) g->d_f.p);
     (void) hipEventRecord(g->ev[1], g->stream);
     g->timed = 1;
     return hipGetLastError();
+}
+
+// The regions of a tree passed to fa2_gpu_repulse_bh: node ranges inside
+// [0, n), links inside [0, nreg) or -1. Returns the first bad region, or -1.
+static int bad_region(const fa2_tree *t, int n)
+{
+    for (int r = 0; r < t->nreg; r++) {
+        const fa2_region &g = t->reg[r];
+        if (g.start < 0 || g.count < 1 || g.start > n - g.count || g.nchild < 0 ||
+            (g.nchild > 0 && (g.child < 0 || g.child > t->nreg - g.nchild)) || g.skip < -1 ||
+            g.skip >= t->nreg)
+            return r;
+    }
+    return -1;
 }
 
 extern "C" int fa2_gpu_repulse_bh(fa2_gpu *g, const fa2_tree *t, double scaling,
@@ -489,6 +603,11 @@ extern "C" int fa2_gpu_repulse_bh(fa2_gpu *g, const fa2_tree *t, double scaling,
     const int nreg = t->nreg;
     if (nreg < 1 || (size_t) nreg > 2 * n) {
         set_message("invalid Barnes-Hut tree (%d regions for %d nodes)", nreg, g->n);
+        return 1;
+    }
+    const int bad = bad_region(t, g->n);
+    if (bad >= 0) {
+        set_message("invalid Barnes-Hut tree (region %d)", bad);
         return 1;
     }
     HipStatus hs;
@@ -513,28 +632,18 @@ extern "C" int fa2_gpu_repulse_bh(fa2_gpu *g, const fa2_tree *t, double scaling,
     t1 = fa2_now();
     g->t_compute += t1 - t0;
     t0 = t1;
-    const size_t rs = g->use_double ? sizeof(double) : sizeof(float);
-    if (hs.ok()) hs.check(hipMemcpyAsync(g->h_f.p, g->d_f.p, 2 * n * rs,
+    if (hs.ok()) hs.check(hipMemcpyAsync(g->h_f.p, g->d_f.p, 2 * n * sizeof(double),
                                          hipMemcpyDeviceToHost, g->stream), "fetch");
     if (hs.ok()) hs.check(hipStreamSynchronize(g->stream), "fetch");
     if (!hs.ok()) {
         set_hip_error("the computation on the GPU failed", hs.err);
         return 1;
     }
-    if (g->use_double) {
-        const double *f = (const double *) g->h_f.p;
-        for (size_t k = 0; k < n; k++) {
-            const int i = t->perm[k];
-            dx[i] += f[2 * k];
-            dy[i] += f[2 * k + 1];
-        }
-    } else {
-        const float *f = (const float *) g->h_f.p;
-        for (size_t k = 0; k < n; k++) {
-            const int i = t->perm[k];
-            dx[i] += (double) f[2 * k];
-            dy[i] += (double) f[2 * k + 1];
-        }
+    const double *f = (const double *) g->h_f.p; // FP64 in both precisions
+    for (size_t k = 0; k < n; k++) {
+        const int i = t->perm[k];
+        dx[i] += f[2 * k];
+        dy[i] += f[2 * k + 1];
     }
     g->t_fetch += fa2_now() - t0;
     return 0;

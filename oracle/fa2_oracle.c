@@ -824,6 +824,86 @@ int orc_repulse_bh(int n, const double *x, const double *y, const double *mass,
     return 0;
 }
 
+/* The FP64 model of the single-precision repulsion offload over the same
+ * Gephi tree (the GPU kernel fa2_bh, src/gpu.c:82-123, with the records of
+ * src/gpu.c:501-519): positions, masses, sizes and region records are
+ * rounded to float32 as the device receives them (cx, cy, mass and
+ * size * size each rounded once); the acceptance test d^2 theta^2 > size^2 is
+ * taken in float32 with each operation rounded on its own (FP_CONTRACT OFF,
+ * src/gpu.c:35), so the model approximates exactly the regions the float32
+ * kernel approximates; every force term is then computed in FP64 from those
+ * float32 values. Node k of the tree order meets its regions in the order of
+ * the stack traversal (last child first). */
+static void c_repulse_node_bh_single(const c_state *s, const g_tree *t, int k, int *stack)
+{
+    const float xf = (float) t->px[k], yf = (float) t->py[k];
+    const double xi = (double) xf, yi = (double) yf;
+    const double mi = (double) (float) t->pm[k];
+    const double si = s->adjust ? (double) (float) t->ps[k] : 0.0;
+    const float th2 = (float) (s->theta * s->theta);
+    double fx = 0.0, fy = 0.0;
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const g_region *r = &t->reg[stack[--sp]];
+        if (r->count > 1) {
+            const float cxf = (float) r->cx, cyf = (float) r->cy;
+            const float txd = xf - cxf, tyd = yf - cyf;
+            if ((txd * txd + tyd * tyd) * th2 > (float) (r->size * r->size)) {
+                const double xd = xi - (double) cxf, yd = yi - (double) cyf;
+                const double f = s->scaling * (mi * (double) (float) r->mass) / (xd * xd + yd * yd);
+                fx += xd * f;
+                fy += yd * f;
+                continue;
+            }
+            if (r->nchild > 0) {
+                for (int c = 0; c < r->nchild; c++) stack[sp++] = r->child + c;
+                continue;
+            }
+        }
+        for (int j = r->start; j < r->start + r->count; j++) {
+            if (j == k) continue;
+            const double xd = xi - (double) (float) t->px[j];
+            const double yd = yi - (double) (float) t->py[j];
+            const double f = c_rep(s, xd * xd + yd * yd, mi * (double) (float) t->pm[j],
+                                   s->adjust ? si + (double) (float) t->ps[j] : 0.0);
+            fx += xd * f;
+            fy += yd * f;
+        }
+    }
+    s->dx[t->perm[k]] += fx;
+    s->dy[t->perm[k]] += fy;
+}
+
+int orc_repulse_bh_single(int n, const double *x, const double *y, const double *mass,
+                          const double *size, double scaling, double theta, double *dx,
+                          double *dy)
+{
+    g_tree t;
+    if (g_tree_alloc(&t, n, size != NULL)) {
+        g_tree_free(&t);
+        return 1;
+    }
+    int *stack = (int *) malloc(4 * (size_t) n * sizeof(int) + 64);
+    if (!stack) {
+        g_tree_free(&t);
+        return 1;
+    }
+    g_tree_build(&t, n, x, y, mass, size, 1);
+    c_state s;
+    memset(&s, 0, sizeof(s));
+    s.n = n;
+    s.adjust = size != NULL;
+    s.scaling = scaling;
+    s.theta = theta;
+    s.dx = dx;
+    s.dy = dy;
+    for (int k = 0; k < n; k++) c_repulse_node_bh_single(&s, &t, k, stack);
+    free(stack);
+    g_tree_free(&t);
+    return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* All-GPU pipeline restated on the CPU: inst/opencl/pipeline.cl             */
 /* ------------------------------------------------------------------------ */

@@ -39,6 +39,11 @@ void orc_repulse_exact(int n, const double *x, const double *y,
 int orc_repulse_bh(int n, const double *x, const double *y, const double *mass,
                    const double *size, double scaling, double theta,
                    int threads, double *dx, double *dy);
+/* FP64 model of the single-precision offload: float32 inputs and records,
+ * float32 acceptance test, FP64 terms (fa2_oracle.c). */
+int orc_repulse_bh_single(int n, const double *x, const double *y, const double *mass,
+                          const double *size, double scaling, double theta, double *dx,
+                          double *dy);
 uint32_t orc_spread16(uint32_t v);
 void orc_radix_sort(int n, const uint32_t *key_in, uint32_t *key_out,
                     uint32_t *perm);

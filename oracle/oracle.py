@@ -41,6 +41,7 @@ def _load():
         "orc_repulse_exact": (None, [c_int, V, V, V, V, c_double, c_int, V, V]),
         "orc_repulse_bh": (c_int, [c_int, V, V, V, V, c_double, c_double,
                                    c_int, V, V]),
+        "orc_repulse_bh_single": (c_int, [c_int, V, V, V, V, c_double, c_double, V, V]),
         "orc_spread16": (c_uint32, [c_uint32]),
         "orc_radix_sort": (None, [c_int, V, V, V]),
         "orc_karras": (None, [c_int, V, V, V, V]),
@@ -144,6 +145,19 @@ def repulse_bh(x, y, mass, scaling, theta, size=None, threads=1):
     if lib.orc_repulse_bh(n, _p(x), _p(y), _p(mass), _p(size), scaling, theta,
                           threads, _p(dx), _p(dy)) != 0:
         raise RuntimeError("orc_repulse_bh failed")
+    return np.stack([dx, dy], 1)
+
+
+def repulse_bh_single(x, y, mass, scaling, theta, size=None):
+    """The FP64 model of the single-precision offload over the same Gephi
+    tree: float32 inputs and records, float32 acceptance test, FP64 terms."""
+    n = len(x)
+    x, y, mass = (np.ascontiguousarray(a, np.float64) for a in (x, y, mass))
+    size = None if size is None else np.ascontiguousarray(size, np.float64)
+    dx, dy = np.zeros(n), np.zeros(n)
+    if lib.orc_repulse_bh_single(n, _p(x), _p(y), _p(mass), _p(size), scaling, theta,
+                                 _p(dx), _p(dy)) != 0:
+        raise RuntimeError("orc_repulse_bh_single failed")
     return np.stack([dx, dy], 1)
 
 

@@ -433,3 +433,106 @@ def test_single_model_takes_the_reference_single_decisions():
     am = O.large_run(500, sub, pos[:500], 2, mode="m", **kw)[1]["force"]
     ad = O.large_run(500, sub, pos[:500], 2, mode="d", **kw)[1]["force"]
     assert np.array_equal(am[0], ad[0])
+
+
+def _walk_single_numpy(t, theta, scaling, fp64_decisions=False):
+    """A second restatement of the single-precision offload model, over the
+    oracle's own Gephi tree: float32 records and acceptance test (numpy
+    float32 scalars, each operation rounded), FP64 terms; the children of a
+    region from the last to the first (src/gpu.c:82-123). With
+    fp64_decisions the test is taken in FP64 on the FP64 records instead."""
+    f32 = np.float32
+    n = len(t["px"])
+    rd, ri = t["reg_d"], t["reg_i"]
+    px, py, pm = (a.astype(f32) for a in (t["px"], t["py"], t["pm"]))
+    th2 = f32(theta * theta)
+    out = np.zeros((n, 2))
+    for k in range(n):
+        xf, yf = px[k], py[k]
+        fx = fy = 0.0
+        stack = [0]
+        while stack:
+            r = stack.pop()
+            start, count, child, nchild = ri[r, :4]
+            if count > 1:
+                cx, cy = f32(rd[r, 1]), f32(rd[r, 2])
+                if fp64_decisions:
+                    ux, uy = float(xf) - rd[r, 1], float(yf) - rd[r, 2]
+                    accept = (ux * ux + uy * uy) * (theta * theta) > rd[r, 3] * rd[r, 3]
+                else:
+                    txd, tyd = f32(xf - cx), f32(yf - cy)
+                    d2 = f32(f32(txd * txd) + f32(tyd * tyd))
+                    accept = f32(d2 * th2) > f32(rd[r, 3] * rd[r, 3])
+                if accept:
+                    if fp64_decisions:
+                        xd, yd = ux, uy
+                    else:
+                        xd, yd = float(xf) - float(cx), float(yf) - float(cy)
+                    g = scaling * (float(pm[k]) * float(f32(rd[r, 0]))) / (xd * xd + yd * yd)
+                    fx += xd * g
+                    fy += yd * g
+                    continue
+                if nchild > 0:
+                    stack.extend(range(child, child + nchild))
+                    continue
+            for j in range(start, start + count):
+                if j == k:
+                    continue
+                xd, yd = float(xf) - float(px[j]), float(yf) - float(py[j])
+                d2 = xd * xd + yd * yd
+                g = scaling * (float(pm[k]) * float(pm[j])) / d2 if d2 > 0 else 0.0
+                fx += xd * g
+                fy += yd * g
+        out[t["perm"][k]] = (fx, fy)
+    return out
+
+
+def _borderline_theta(t):
+    """A theta at which the float32 and the FP64 acceptance tests disagree
+    for some node of the tree and some child of the root (which every node
+    tests): found by stepping theta across the FP64 threshold."""
+    f32 = np.float32
+    rd, ri = t["reg_d"], t["reg_i"]
+    child, nchild = ri[0, 2], ri[0, 3]
+    px, py = t["px"].astype(f32), t["py"].astype(f32)
+    for r in range(child, child + nchild):
+        if ri[r, 1] < 2:
+            continue
+        cx, cy, s2 = f32(rd[r, 1]), f32(rd[r, 2]), rd[r, 3] * rd[r, 3]
+        for k in np.argsort(np.abs(px.astype(float) - rd[r, 1]))[:200]:
+            ux, uy = float(px[k]) - rd[r, 1], float(py[k]) - rd[r, 2]
+            th = np.sqrt(s2 / (ux * ux + uy * uy))
+            for _ in range(64):
+                txd, tyd = f32(px[k] - cx), f32(py[k] - cy)
+                a = f32(f32(f32(txd * txd) + f32(tyd * tyd)) * f32(th * th)) > f32(s2)
+                if a != ((ux * ux + uy * uy) * (th * th) > s2):
+                    return th
+                th = np.nextafter(th, np.inf)
+    raise AssertionError("no borderline theta found")
+
+
+def test_single_offload_model():
+    """orc_repulse_bh_single (the FP64 model of the single-precision offload)
+    against the numpy restatement above on the same tree, within FP64
+    rounding, at theta 1.2 and at a theta where float32 and FP64 decisions
+    disagree for one (node, region) pair: there the FP64-decision variant
+    differs by far more than rounding, so a model that decided in FP64
+    would fail. theta = 0 is the exact all-pairs sum on the float32
+    inputs."""
+    rng = np.random.default_rng(8)
+    n = 1500
+    x, y = rng.uniform(-300, 300, n), rng.uniform(-300, 300, n)
+    x[:60], y[:60] = x[np.arange(60) % 6], y[np.arange(60) % 6]  # coincident nodes
+    m = 1.0 + rng.integers(0, 20, n)
+    t = O.gephi_tree(x, y, m)
+    for theta in (1.2, _borderline_theta(t)):
+        got = O.repulse_bh_single(x, y, m, 2.0, theta)
+        want = _walk_single_numpy(t, theta, 2.0)
+        assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max(), theta
+    other = _walk_single_numpy(t, theta, 2.0, fp64_decisions=True)
+    node_err = np.abs(got - other).max(1) / np.abs(got).max(1)
+    assert node_err.max() > 1e-4
+    xf = x.astype(np.float32).astype(np.float64)
+    yf = y.astype(np.float32).astype(np.float64)
+    exact = O.repulse_exact(xf, yf, m, 2.0)
+    assert normwise(O.repulse_bh_single(x, y, m, 2.0, 0.0), exact) <= 1e-12
