@@ -484,6 +484,79 @@ k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     }
 }
 
+// Stable LSD radix sort of n (key, value) pairs on the low `bits` bits of
+// the keys (ceil(bits / 8) passes of 8 bits), with the kernels above over a
+// single segment: the pairs are one graph of ceil(n / 2048) tiles. Input in
+// k[0], v[0]; the sorted pairs end in k[*out], v[*out]. The tile table and
+// counts are allocated in the order of `ast` (synchronously when null); the
+// kernels run on `st`, which the caller synchronises before it frees k, v.
+int radix_sort_pairs_u32(hipStream_t st, hipStream_t ast, int n, int bits, uint32_t *k[2],
+                         uint32_t *v[2], int *out)
+{
+    *out = 0;
+    if (n <= 1) return 0;
+    const int T = (int) ceil_div((size_t) n, kTile);
+    std::vector<Tile> tiles(T);
+    for (int j = 0; j < T; j++) tiles[j] = Tile{0, j * kTile, std::min(n, (j + 1) * kTile), j};
+    GraphDesc gd;
+    memset(&gd, 0, sizeof(gd));
+    gd.n = n;
+    gd.ntiles = T;
+    GraphState gs;
+    memset(&gs, 0, sizeof(gs));
+    const int zero = 0;
+    HipStatus hs;
+    DevBuf<Tile> d_tiles;
+    DevBuf<GraphDesc> d_gd;
+    DevBuf<GraphState> d_gs;
+    DevBuf<int> d_glist;
+    DevBuf<uint32_t> d_hist, d_rowtot;
+    hs.check(d_tiles.alloc(T, ast), "alloc");
+    hs.check(d_gd.alloc(1, ast), "alloc");
+    hs.check(d_gs.alloc(1, ast), "alloc");
+    hs.check(d_glist.alloc(1, ast), "alloc");
+    hs.check(d_hist.alloc((size_t) 256 * T, ast), "alloc");
+    hs.check(d_rowtot.alloc(256, ast), "alloc");
+    if (hs.ok())
+        hs.check(hipMemcpyAsync(d_tiles.p, tiles.data(), T * sizeof(Tile), hipMemcpyHostToDevice,
+                                st), "upload");
+    if (hs.ok())
+        hs.check(hipMemcpyAsync(d_gd.p, &gd, sizeof(gd), hipMemcpyHostToDevice, st), "upload");
+    if (hs.ok())
+        hs.check(hipMemcpyAsync(d_gs.p, &gs, sizeof(gs), hipMemcpyHostToDevice, st), "upload");
+    if (hs.ok())
+        hs.check(hipMemcpyAsync(d_glist.p, &zero, sizeof(int), hipMemcpyHostToDevice, st),
+                 "upload");
+    const int passes = std::max(1, (bits + 7) / 8);
+    int in = 0;
+    for (int pass = 0; hs.ok() && pass < passes; pass++) {
+        k_sort_upsweep<<<T, kSortThreads, 0, st>>>(d_tiles.p, d_gd.p, d_gs.p, k[in], 8 * pass,
+                                                    d_hist.p);
+        if (T <= kRowThreadTiles)
+            k_sort_rowscan_thread<<<1, 256, 0, st>>>(256, d_glist.p, d_gd.p, d_gs.p, d_hist.p,
+                                                     d_rowtot.p);
+        else if (T <= 64)
+            k_sort_rowscan<<<64, 256, 0, st>>>(256, d_glist.p, d_gd.p, d_gs.p, d_hist.p,
+                                               d_rowtot.p);
+        else
+            k_sort_rowscan_block<<<256, 256, 0, st>>>(d_glist.p, d_gd.p, d_gs.p, d_hist.p,
+                                                      d_rowtot.p);
+        k_sort_downsweep<float4, false><<<T, kSortThreads, 0, st>>>(
+            d_tiles.p, d_gd.p, d_gs.p, k[in], v[in], 8 * pass, d_hist.p, d_rowtot.p, k[in ^ 1],
+            v[in ^ 1], nullptr, nullptr);
+        hs.check(hipGetLastError(), "launch");
+        in ^= 1;
+    }
+    // the host arrays above and the device tables must outlive the kernels
+    if (hs.ok()) hs.check(hipStreamSynchronize(st), "sort");
+    if (!hs.ok()) {
+        set_hip_error("the sort on the GPU failed", hs.err);
+        return 1;
+    }
+    *out = in;
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Tree: gather, Karras radix tree, bottom-up summaries (pipeline.cl:195-323)
 // ---------------------------------------------------------------------------

@@ -31,4 +31,10 @@ int ingest_graphs(hipStream_t st, hipStream_t ast, int N, const std::vector<Inge
                   DevBuf<int> &inc_off, DevBuf<int> &inc_node, DevBuf<R> &inc_coef,
                   std::vector<int> &h_inc_off);
 
+// Stable LSD radix sort of n (key, value) pairs on the low `bits` bits of
+// the keys, by the kernels of the batch sort over one segment (engine.hip):
+// k[0], v[0] in, k[*out], v[*out] out. 0 on success, 1 with a message.
+int radix_sort_pairs_u32(hipStream_t st, hipStream_t ast, int n, int bits, uint32_t *k[2],
+                         uint32_t *v[2], int *out);
+
 } // namespace fa2

@@ -10,13 +10,16 @@
 //              row to, both with the coefficient
 //   inc_coef = coef * w[e] / mass[from[e]] with dissuade_hubs, else coef * w[e].
 //
-// The rows come from one stable radix sort (rocPRIM) of the 2m incidence
-// entries by global node id: entries are generated in edge order, so a
-// stable sort keeps each row in edge order. Degrees are integer atomics, so
-// every output is deterministic and equal, bit for bit, to the host
-// preprocessing.
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
+// The rows come from one stable radix sort of the 2m incidence entries by
+// global node id (the batch sort's own kernels, over one segment): entries
+// are generated in edge order, so a stable sort keeps each row in edge
+// order. The row offsets are an exclusive scan of the per-node counts, by
+// blocks: each block scans its 1024 counts in LDS and leaves its total, one
+// workgroup scans the totals, and each block adds the sum before it.
+// Degrees are integer atomics, so every output is deterministic and equal,
+// bit for bit, to the host preprocessing.
+#include <algorithm>
+#include <climits>
 
 #include "common.h"
 #include "ingest.h"
@@ -91,6 +94,80 @@ __global__ void k_ingest_fill(long long E, int G, const IngestGraph *__restrict_
     inc_coef[k] = (R) c;
 }
 
+constexpr int kScanThreads = 256, kScanItems = 4;
+constexpr int kScanBlock = kScanThreads * kScanItems; // counts per block
+
+// Exclusive scan of 256 values, one per thread of the workgroup; returns
+// the thread's exclusive prefix and sets `total`.
+__device__ inline long long block_exclusive(long long v, long long *lds, long long &total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    long long s = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long u = __shfl_up(s, o, 64);
+        if (lane >= o) s += u;
+    }
+    if (lane == 63) lds[w] = s;
+    __syncthreads();
+    long long before = 0;
+    for (int k = 0; k < w; k++) before += lds[k];
+    total = lds[0] + lds[1] + lds[2] + lds[3];
+    __syncthreads();
+    return before + s - v;
+}
+
+// out[i] = sum of in[0..i) within each block of kScanBlock values, and the
+// block's total in sums[block].
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_blocks(int n, const int *__restrict__ in, int *__restrict__ out,
+              long long *__restrict__ sums)
+{
+    __shared__ long long lds[4];
+    const int base = blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    long long t = 0;
+#pragma unroll
+    for (int u = 0; u < kScanItems; u++) {
+        v[u] = base + u < n ? in[base + u] : 0;
+        t += v[u];
+    }
+    long long total;
+    long long run = block_exclusive(t, lds, total);
+#pragma unroll
+    for (int u = 0; u < kScanItems; u++) {
+        if (base + u < n) out[base + u] = (int) run;
+        run += v[u];
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// sums[b] = sum of the totals of blocks 0..b-1 (one workgroup).
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_sums(int nb, long long *__restrict__ sums)
+{
+    __shared__ long long lds[4];
+    long long carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += kScanThreads) {
+        const int b = b0 + (int) threadIdx.x;
+        const long long v = b < nb ? sums[b] : 0;
+        long long total;
+        const long long ex = block_exclusive(v, lds, total);
+        if (b < nb) sums[b] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_add(int n, const long long *__restrict__ sums, int *__restrict__ out)
+{
+    const int base = blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
+    const long long add = sums[blockIdx.x];
+#pragma unroll
+    for (int u = 0; u < kScanItems; u++)
+        if (base + u < n) out[base + u] = (int) (out[base + u] + add);
+}
+
 int bits_for(uint32_t v)
 {
     int b = 1;
@@ -113,7 +190,7 @@ int ingest_graphs(hipStream_t st, hipStream_t ast, int N, const std::vector<Inge
         M += edges[g]->m;
         any_w = any_w || edges[g]->w;
     }
-    if (2 * M >= (long long) UINT32_MAX) {
+    if (2 * M >= (long long) INT_MAX) {
         set_message("too many edges for one batch");
         return 1;
     }
@@ -174,30 +251,25 @@ int ingest_graphs(hipStream_t st, hipStream_t ast, int N, const std::vector<Inge
         hs.check(hipGetLastError(), "launch");
     }
     // rows: exclusive sum of the incidence counts, then the stable sort
-    size_t scan_bytes = 0, sort_bytes = 0;
-    if (hs.ok())
-        hs.check(rocprim::exclusive_scan(nullptr, scan_bytes, d_cnt.p, inc_off.p, 0,
-                                         (size_t) N + 1, rocprim::plus<int>(), st),
-                 "scan");
-    const int kbits = bits_for((uint32_t) N);
-    if (hs.ok() && E2 > 0)
-        hs.check(rocprim::radix_sort_pairs(nullptr, sort_bytes, d_key[0].p, d_key[1].p,
-                                           d_val[0].p, d_val[1].p, E2, 0, kbits, st),
-                 "sort");
-    DevBuf<char> tmp;
-    if (hs.ok()) hs.check(tmp.alloc(std::max(scan_bytes, sort_bytes), ast), "alloc");
-    if (hs.ok())
-        hs.check(rocprim::exclusive_scan(tmp.p, scan_bytes, d_cnt.p, inc_off.p, 0,
-                                         (size_t) N + 1, rocprim::plus<int>(), st),
-                 "scan");
-    if (hs.ok() && E2 > 0)
-        hs.check(rocprim::radix_sort_pairs(tmp.p, sort_bytes, d_key[0].p, d_key[1].p,
-                                           d_val[0].p, d_val[1].p, E2, 0, kbits, st),
-                 "sort");
+    const int nscan = N + 1, nb = (int) ceil_div((size_t) nscan, kScanBlock);
+    DevBuf<long long> d_sums;
+    hs.check(d_sums.alloc((size_t) nb, ast), "alloc");
+    if (hs.ok()) {
+        k_scan_blocks<<<nb, kScanThreads, 0, st>>>(nscan, d_cnt.p, inc_off.p, d_sums.p);
+        k_scan_sums<<<1, kScanThreads, 0, st>>>(nb, d_sums.p);
+        k_scan_add<<<nb, kScanThreads, 0, st>>>(nscan, d_sums.p, inc_off.p);
+        hs.check(hipGetLastError(), "launch");
+    }
+    int sorted = 0;
+    if (hs.ok() && E2 > 1) {
+        uint32_t *kk[2] = {d_key[0].p, d_key[1].p}, *vv[2] = {d_val[0].p, d_val[1].p};
+        if (radix_sort_pairs_u32(st, ast, (int) E2, bits_for((uint32_t) N), kk, vv, &sorted))
+            return 1;
+    }
     if (hs.ok() && E2 > 0) {
         k_ingest_fill<R><<<(unsigned) ((E2 + 255) / 256), 256, 0, st>>>(
             (long long) E2, G, d_gr.p, d_from.p, d_to.p, any_w ? d_w.p : nullptr, d_deg.p,
-            d_val[1].p, inc_off.p, N, inc_node.p, inc_coef.p);
+            d_val[sorted].p, inc_off.p, N, inc_node.p, inc_coef.p);
         hs.check(hipGetLastError(), "launch");
     }
     int bad = 0;

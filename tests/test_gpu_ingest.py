@@ -3,17 +3,20 @@ mass = 1 + degree, incidence lists in edge order without self-loops, and
 the attraction coefficients of C_forceatlas2_large
 (src/forceatlas2.c:866-902).
 
-Gates: the device's mass, row offsets and incidence nodes equal the host
-preprocessing (api.prepare_large, itself pinned bit for bit to the oracle's
-orc_large_prepare) exactly; the coefficients exactly in double precision and
-as their float32 rounding in single; and layouts from either preprocessing
-are bit-identical.
+Gates: the device's mass, row offsets and incidence nodes equal the
+oracle's restatement of the reference preprocessing (orc_large_prepare) and
+the host preprocessing (api.prepare_large) exactly; the coefficients exactly
+in double precision and as their float32 rounding in single; and layouts
+from either preprocessing are bit-identical. The cases reach every path of
+the device sort (one to three passes of 8-bit digits; tile counts scanned by
+a thread, a wave and a workgroup) and of the block scan of the row counts.
 """
 import numpy as np
 import pytest
 
 from forceatlas2_rc_amd import Batch, GPUError, Server, graphs
 from forceatlas2_rc_amd.api import prepare_large
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -44,7 +47,9 @@ def test_device_preprocessing_equals_host(precision):
         else:
             e, w = np.zeros((0, 2), np.int32), np.zeros(0)
         gs.append(_graph(n, e, w, outbound, k))
-        refs.append(prepare_large(n, e[:, 0], e[:, 1], w, outbound))
+        refs.append(O.large_prepare(n, e, w, outbound))
+        for a, c in zip(refs[-1], prepare_large(n, e[:, 0], e[:, 1], w, outbound)):
+            assert np.array_equal(a, c)
     with Batch(gs, precision=precision, ingest="device") as b:
         for g, (mass, off, node, coef) in enumerate(refs):
             dm, doff, dnode, dcoef = b.csr(g)
@@ -59,7 +64,7 @@ def test_device_preprocessing_at_scale():
     """R-MAT scale 17 (hubs of thousands of incidences, many isolated
     nodes), with dissuade_hubs."""
     n, e = graphs.rmat(17, 1_000_000, seed=3)
-    mass, off, node, coef = prepare_large(n, e[:, 0], e[:, 1], np.ones(len(e)), True)
+    mass, off, node, coef = O.large_prepare(n, e, None, True)
     g = _graph(n, e, None, True, 5)
     with Batch([g], precision="double", ingest="device") as b:
         dm, doff, dnode, dcoef = b.csr(0)
