@@ -36,11 +36,14 @@ Beside it, in the same line:
   * "serving": a Poisson stream of layout requests (preferential-attachment
     or block-model graphs of 1k..1M nodes, log-uniform; mean degree 4..16;
     100..1000 iterations; linlog, prevent_overlap with node sizes in [1, 3],
-    5% pinned nodes and double precision mixed in) served by the layout
-    server, against the same requests solved one at a time; p50/p99
-    latency and layouts/s, at the offered rate and again at 3 requests/s
-    (below the server's capacity). Both start from prepared incidence lists
-    and include the upload and the download of every graph.
+    5% pinned nodes and double precision mixed in; 200 requests) served
+    by the layout server, against the same requests solved one at a time;
+    p50/p99 latency and layouts/s, at the offered rate and, for the first
+    40 requests, at 3 requests/s (below the server's capacity); and the
+    ceiling of the speedup that the size classes allow (each class at the
+    rate of one batch of all its requests). Both start from prepared
+    incidence lists and include the upload and the download of every
+    graph.
 """
 import argparse
 import json
@@ -319,11 +322,46 @@ def queue_latency(arrive, svc):
     return np.array(lat)
 
 
+SIZE_CLASSES = ((1_000, 10_000), (10_000, 100_000), (100_000, 1 << 30))
+
+
+def class_ceiling(reqs, svc, iters=40):
+    """The speedup no scheduler of these requests can pass while each size
+    class runs at the rate of its own best batch: per (size class,
+    precision), the node-iterations per second of one batch holding all of
+    that class's requests (timed over `iters` iterations after 2 warm-up
+    ones), against the same requests' rate one at a time (their measured
+    service times). Returns the per-class rates and the ceiling."""
+    from forceatlas2_rc_amd import Batch
+    out, t_best = [], 0.0
+    for lo, hi in SIZE_CLASSES:
+        for prec in ("single", "double"):
+            sel = [k for k, (g, p, _) in enumerate(reqs) if lo <= g["n"] < hi and p == prec]
+            if not sel:
+                continue
+            work = sum(reqs[k][0]["n"] * reqs[k][0]["iterations"] for k in sel)
+            alone_s = float(sum(svc[k] for k in sel))
+            gs = [dict(reqs[k][0], iterations=1 << 30) for k in sel]
+            with Batch(gs, precision=prec, autotune=False) as b:
+                b.run(2)
+                b.sync()
+                t0 = time.perf_counter()
+                b.run(iters)
+                b.sync()
+                dt = time.perf_counter() - t0
+            rate = sum(g["n"] for g in gs) * iters / dt
+            t_best += work / rate
+            out.append(dict(nodes=[lo, hi], precision=prec, requests=len(sel),
+                            node_iterations=work, batched_node_iters_per_s=rate,
+                            one_at_a_time_node_iters_per_s=work / alone_s))
+    return out, float(sum(svc)) / t_best
+
+
 def serving_leg(args):
     """A Poisson stream of layout requests through the server, against the
     same requests solved one at a time: at the offered rate (above capacity,
-    so the stream saturates the server) and, with the same requests, at a
-    rate below capacity."""
+    so the stream saturates the server) and, with the first requests, at a
+    rate below capacity; then the size-class ceiling of the speedup."""
     from forceatlas2_rc_amd import Batch
     t0 = time.time()
     reqs, arrive = serving_requests(args)
@@ -335,8 +373,9 @@ def serving_leg(args):
         b.run(2)
         b.sync()
     out, span, lat, stats = serve_stream(reqs, arrive, args)
-    arrive_sub = arrive * (args.serve_rate / args.serve_sub_rate)
-    out_sub, span_sub, lat_sub, stats_sub = serve_stream(reqs, arrive_sub, args)
+    n_sub = min(len(reqs), args.serve_sub_requests)
+    arrive_sub = arrive[:n_sub] * (args.serve_rate / args.serve_sub_rate)
+    out_sub, span_sub, lat_sub, stats_sub = serve_stream(reqs[:n_sub], arrive_sub, args)
     # one at a time, in arrival order
     svc, same = [], 0
     for k, (g, prec, _) in enumerate(reqs):
@@ -346,11 +385,16 @@ def serving_leg(args):
             b.sync()
             p = b.positions(0)
         svc.append(time.perf_counter() - t1)
-        same += int(np.array_equal(p, out[k][0]) and np.array_equal(p, out_sub[k][0]))
+        same += int(np.array_equal(p, out[k][0]) and
+                    (k >= n_sub or np.array_equal(p, out_sub[k][0])))
+        if k % 50 == 49:
+            log(f"one at a time: {k + 1} requests")
     svc = np.array(svc)
     lat1 = queue_latency(arrive, svc)
-    lat1_sub = queue_latency(arrive_sub, svc)
+    lat1_sub = queue_latency(arrive_sub, svc[:n_sub])
     log(f"one at a time: {svc.sum():.2f} s of service")
+    classes, ceiling = class_ceiling(reqs, svc)
+    log(f"size-class ceiling of the speedup: {ceiling:.2f}")
     pct = lambda a, q: float(np.percentile(a, q))
     return dict(workload=(f"Poisson stream, {args.serve_rate:g} requests/s, {len(reqs)} "
                           f"requests; preferential-attachment or SBM graphs of 1000.."
@@ -359,17 +403,21 @@ def serving_leg(args):
                           "pinned, dissuade_hubs and double precision mixed in"),
                 requests=len(reqs), nodes=n_tot, node_iterations=work,
                 layouts_per_s=len(reqs) / span, p50_s=pct(lat, 50), p99_s=pct(lat, 99),
+                node_iters_per_s=work / span,
                 one_at_a_time=dict(layouts_per_s=len(reqs) / svc.sum(),
                                    p50_s=pct(lat1, 50), p99_s=pct(lat1, 99),
-                                   service_s=float(svc.sum())),
+                                   service_s=float(svc.sum()),
+                                   node_iters_per_s=work / svc.sum()),
                 speedup=(len(reqs) / span) / (len(reqs) / svc.sum()),
-                batches=stats["batches"], max_batches_running=stats["peak_running"],
-                below_capacity=dict(rate=args.serve_sub_rate, span_s=span_sub,
-                                    layouts_per_s=len(reqs) / span_sub,
+                size_class_ceiling=dict(speedup=ceiling, classes=classes),
+                batches=stats["batches"], refills=stats["refills"],
+                max_batches_running=stats["peak_running"],
+                below_capacity=dict(rate=args.serve_sub_rate, requests=n_sub, span_s=span_sub,
+                                    layouts_per_s=n_sub / span_sub,
                                     p50_s=pct(lat_sub, 50), p99_s=pct(lat_sub, 99),
                                     one_at_a_time_p50_s=pct(lat1_sub, 50),
                                     one_at_a_time_p99_s=pct(lat1_sub, 99),
-                                    batches=stats_sub["batches"],
+                                    batches=stats_sub["batches"], refills=stats_sub["refills"],
                                     max_batches_running=stats_sub["peak_running"]),
                 identical_results=f"{same}/{len(reqs)}")
 
@@ -396,7 +444,9 @@ def main():
     ap.add_argument("--no-batch-alone", action="store_true",
                     help="skip laying out each graph of the batch alone")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-sub-requests", type=int, default=40,
+                    help="requests of the stream replayed below capacity")
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-sub-rate", type=float, default=3.0)
     ap.add_argument("--serve-max-nodes", type=int, default=1_000_000)

@@ -18,9 +18,17 @@
 // skipped from then on). A batch's memory, allocated in its stream's order,
 // is released when its last request is done.
 //
+// Continuous batching: when every stream is taken and requests wait, a
+// batch that has finished some of its requests takes waiting ones in at its
+// next chunk boundary. Its remaining requests' state (positions, previous
+// force, speed and speed efficiency, iterations done) is read back, the
+// batch is rebuilt on its stream with them and the newcomers that fit in
+// the memory budget, and each continuing graph resumes from its state.
+//
 // A request's result does not depend on the other requests: every kernel of
 // the engine computes a graph from that graph's data alone, in a fixed
-// order, so a layout is bit-identical alone and inside any batch.
+// order, and an iteration depends only on that state, so a layout is
+// bit-identical alone, inside any batch, and across a rebuild.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -56,6 +64,11 @@ struct Job {
     size_t bytes = 0;
     int state = kQueued;
     std::string error;
+    // iterations applied in earlier batches, and the state they left
+    // (positions in pos; previous force; speed and efficiency)
+    int done_before = 0;
+    std::vector<double> fold;
+    double speed = 1.0, eff = 1.0;
 };
 
 struct Running {
@@ -85,14 +98,150 @@ struct fa2_server {
     std::thread worker;
     bool stop = false;
     // statistics
-    long long batches = 0, admitted = 0;
+    long long batches = 0, admitted = 0, refills = 0;
     int peak_running = 0;
 
     void loop();
     bool admit(std::unique_lock<std::mutex> &lk);
     bool advance(Running *r);
     void finish_batch(Running *r);
+    std::string open_engine(Running *r, const std::vector<Job *> &jobs, int use_double,
+                            bool has_edges);
+    bool refill(Running *r, const std::vector<int> &done);
 };
+
+// A new engine for `jobs` on r's stream: each job continues from its
+// iterations done and saved state. An empty string on success.
+std::string fa2_server::open_engine(Running *r, const std::vector<Job *> &jobs, int use_double,
+                                    bool has_edges)
+{
+    const size_t G = jobs.size();
+    std::vector<fa2_full_params> ps(G);
+    std::vector<EngineGraph> gr(G);
+    r->longest = 0;
+    for (size_t k = 0; k < G; k++) {
+        ps[k] = jobs[k]->p;
+        ps[k].iterations = jobs[k]->p.iterations - jobs[k]->done_before;
+        r->longest = std::max(r->longest, ps[k].iterations);
+        gr[k] = EngineGraph{&ps[k], jobs[k]->pos.data(), has_edges ? &jobs[k]->edges : nullptr};
+    }
+    r->e = make_engine(use_double);
+    if (!r->e) return "out of memory";
+    r->e->device = device;
+    r->e->stream = r->st;
+    r->e->async_alloc = 1;
+    // batches already run side by side: one stream each (a second one per
+    // batch would share the hardware queues between batches), and the
+    // default launch shapes (the autotuner synchronises the device and
+    // would hold the worker between chunks)
+    r->e->overlap = 0;
+    if (r->e->open(gr.data(), (int) G) != 0) return fa2_gpu_error();
+    for (size_t k = 0; k < G; k++) {
+        const Job *j = jobs[k];
+        if (j->done_before > 0 &&
+            r->e->set_state((int) k, j->pos.data(), j->fold.data(), j->speed, j->eff) != 0)
+            return fa2_gpu_error();
+    }
+    r->jobs = jobs;
+    r->finished.assign(G, 0);
+    r->target = 0;
+    return std::string();
+}
+
+// At a chunk boundary of r, when no stream is free and requests of r's
+// precision and kind wait: those that fit in the budget, with what r's
+// finished requests give back, join r's remaining ones in a rebuilt batch
+// on r's stream. A rebuild reads back and uploads again the remaining
+// requests, so it is made only when some of them finished or when the
+// newcomers hold at least a quarter as many nodes as the remaining ones.
+// False when nothing joined (r goes on as it is); on a failure its
+// requests fail.
+bool fa2_server::refill(Running *r, const std::vector<int> &done)
+{
+    const int use_double = r->jobs[0]->p.use_double;
+    const bool has_edges = r->jobs[0]->has_edges;
+    std::vector<Job *> keep;
+    std::vector<int> keep_k;
+    size_t keep_bytes = 0;
+    long long keep_nodes = 0;
+    for (size_t k = 0; k < r->jobs.size(); k++)
+        if (!r->finished[k]) {
+            keep.push_back(r->jobs[k]);
+            keep_k.push_back((int) k);
+            keep_bytes += r->jobs[k]->bytes;
+            keep_nodes += r->jobs[k]->p.n;
+        }
+    const bool slot_free = keep.size() < r->jobs.size();
+    std::vector<Job *> take;
+    size_t take_bytes = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (queue.empty() || !free_streams.empty()) return false;
+        size_t avail = budget - (used - r->bytes) - keep_bytes;
+        long long take_nodes = 0;
+        std::deque<Job *> rest;
+        for (Job *j : queue) {
+            if (j->p.use_double != use_double || j->has_edges != has_edges || j->bytes > avail) {
+                rest.push_back(j);
+                continue;
+            }
+            avail -= j->bytes;
+            take_nodes += j->p.n;
+            take_bytes += j->bytes;
+            take.push_back(j);
+        }
+        if (take.empty() || (!slot_free && 4 * take_nodes < keep_nodes)) return false;
+        queue.swap(rest);
+        used = used - r->bytes + keep_bytes + take_bytes;
+        const double t = now_s();
+        for (Job *j : take) {
+            j->t_admit = t;
+            j->state = kRunning;
+        }
+        admitted += (long long) take.size();
+        refills++;
+    }
+    r->bytes = keep_bytes + take_bytes;
+    // the remaining requests' state
+    std::string err;
+    for (size_t q = 0; q < keep.size() && err.empty(); q++) {
+        Job *j = keep[q];
+        const int k = keep_k[q];
+        const size_t n = (size_t) j->p.n;
+        double sp[4];
+        j->fold.resize(2 * n);
+        if (r->e->positions(k, j->pos.data()) != 0 || r->e->forces(k, j->fold.data()) != 0 ||
+            r->e->speed_state(k, sp) != 0) {
+            err = fa2_gpu_error();
+            break;
+        }
+        j->speed = sp[2];
+        j->eff = sp[3];
+        j->done_before += done[k];
+    }
+    delete r->e; // waits for its stream, frees in the stream's order
+    r->e = nullptr;
+    std::vector<Job *> all = keep;
+    all.insert(all.end(), take.begin(), take.end());
+    if (err.empty()) err = open_engine(r, all, use_double, has_edges);
+    if (err.empty()) {
+        r->target = std::min(chunk, r->longest);
+        if (r->e->run(r->target) != 0 || hipEventRecord(r->ev, r->st) != hipSuccess)
+            err = fa2_gpu_error();
+    }
+    if (!err.empty()) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (Job *j : all) {
+            j->state = kFailed;
+            j->error = err;
+            j->t_done = now_s();
+        }
+        r->jobs = all;
+        r->finished.assign(all.size(), 1);
+        cv_done.notify_all();
+    }
+    return true;
+}
 
 // One new batch from the requests at the head of the queue that fit.
 bool fa2_server::admit(std::unique_lock<std::mutex> &lk)
@@ -130,39 +279,13 @@ bool fa2_server::admit(std::unique_lock<std::mutex> &lk)
 
     // build the batch without holding the lock: the upload takes a while
     Running *r = new Running();
-    r->jobs = take;
-    r->finished.assign(take.size(), 0);
     r->st = st;
     r->bytes = bytes;
-    std::vector<fa2_full_params> ps(take.size());
-    std::vector<const double *> pos(take.size());
     const double t = now_s();
-    for (size_t k = 0; k < take.size(); k++) {
-        ps[k] = take[k]->p;
-        pos[k] = take[k]->pos.data();
-        take[k]->t_admit = t;
-        r->longest = std::max(r->longest, take[k]->p.iterations);
-    }
-    std::string err;
-    r->e = make_engine(use_double);
-    if (!r->e) {
-        err = "out of memory";
-    } else {
-        r->e->device = device;
-        r->e->stream = st;
-        r->e->async_alloc = 1;
-        // batches already run side by side: one stream each (a second one
-        // per batch would share the hardware queues between batches), and the
-        // default launch shapes (the autotuner synchronises the device and
-        // would hold the worker between chunks)
-        r->e->overlap = 0;
-        std::vector<EngineGraph> gr(take.size());
-        for (size_t k = 0; k < take.size(); k++)
-            gr[k] = EngineGraph{&ps[k], pos[k], has_edges ? &take[k]->edges : nullptr};
-        if (r->e->open(gr.data(), (int) take.size()) != 0) err = fa2_gpu_error();
-        else if (hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
-            err = "could not create a HIP event";
-    }
+    for (Job *j : take) j->t_admit = t;
+    std::string err = open_engine(r, take, use_double, has_edges);
+    if (err.empty() && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
+        err = "could not create a HIP event";
     if (err.empty()) {
         r->target = std::min(chunk, r->longest);
         if (r->e->run(r->target) != 0 || hipEventRecord(r->ev, st) != hipSuccess)
@@ -221,7 +344,7 @@ bool fa2_server::advance(Running *r)
             r->finished[k] = 1;
             continue;
         }
-        if (done[k] < j->p.iterations) {
+        if (j->done_before + done[k] < j->p.iterations) {
             all = false;
             continue;
         }
@@ -241,6 +364,11 @@ bool fa2_server::advance(Running *r)
     }
     cv_done.notify_all();
     if (all || !err.empty()) return true;
+    if (refill(r, done)) {
+        for (char f : r->finished)
+            if (!f) return false;
+        return true; // the rebuild failed
+    }
     const int step = std::min(chunk, r->longest - r->target);
     r->target += step;
     if (r->e->run(step) != 0 || hipEventRecord(r->ev, r->st) != hipSuccess) {
@@ -449,7 +577,7 @@ extern "C" int fa2_serve_wait(fa2_server *s, int job, double *pos, double times[
     return 0;
 }
 
-extern "C" int fa2_serve_stats(fa2_server *s, double out[4])
+extern "C" int fa2_serve_stats(fa2_server *s, double out[5])
 {
     if (!s) return 1;
     std::lock_guard<std::mutex> lk(s->mu);
@@ -457,6 +585,7 @@ extern "C" int fa2_serve_stats(fa2_server *s, double out[4])
     out[1] = (double) s->admitted;
     out[2] = (double) s->peak_running;
     out[3] = (double) s->budget;
+    out[4] = (double) s->refills;
     return 0;
 }
 

@@ -237,7 +237,10 @@ typedef struct fa2_server fa2_server;
  * footprint (budget_bytes, or 80% of the free memory when <= 0) into
  * batches of one precision each, up to max_batches (1..16) of them running
  * at once on their own streams, each advanced `chunk` iterations at a
- * time. NULL on error. */
+ * time. When every stream is taken, waiting requests join a running batch
+ * at a chunk boundary after some of its requests finished (the batch is
+ * rebuilt with its remaining requests' state and the newcomers). NULL on
+ * error. */
 fa2_server *fa2_serve_open(int device, double budget_bytes, int max_batches, int chunk);
 
 /* Queue one layout (arguments as for fa2_full_run, use_double choosing its
@@ -262,8 +265,8 @@ int fa2_serve_status(fa2_server *s, int job);
 int fa2_serve_wait(fa2_server *s, int job, double *pos, double times[3]);
 
 /* out = (batches formed, requests admitted, most batches running at once,
- * memory budget in bytes). */
-int fa2_serve_stats(fa2_server *s, double out[4]);
+ * memory budget in bytes, batches rebuilt to take waiting requests in). */
+int fa2_serve_stats(fa2_server *s, double out[5]);
 
 /* Finish the queued and running requests, then release the server. */
 void fa2_serve_close(fa2_server *s);

@@ -124,3 +124,29 @@ def test_server_argument_errors():
         assert lib.fa2_serve_wait(srv.h, 999, None, None) == 1
         with pytest.raises(GPUError):
             srv.submit(_graph(100, 41, iterations=0))
+
+
+def test_waiting_requests_join_a_running_batch():
+    """Continuous batching: with one stream, requests submitted while a
+    batch runs join it at a chunk boundary once one of its requests has
+    finished (the batch is rebuilt with the remaining request's state);
+    requests of the other precision wait for the stream. Every result equals
+    the layout alone bit for bit, the continuing one included."""
+    long_ = _graph(3000, 51, iterations=400, linlog=True)
+    short = _graph(800, 52, iterations=10)
+    late = [_graph(1500, 53, iterations=35, dissuade_hubs=True),
+            _graph(600, 54, iterations=60, adjust=True, size=1.5),
+            _graph(2200, 55, iterations=500)]
+    other = _graph(900, 56, iterations=20)
+    with Server(max_batches=1, chunk=5) as srv:
+        i_long, i_short = srv.submit(long_), srv.submit(short)
+        srv.result(i_short)  # the batch runs on; a slot is free
+        ids = [srv.submit(g) for g in late]
+        i_other = srv.submit(other, "double")
+        res = [srv.result(j)[0] for j in [i_long] + ids]
+        res_other = srv.result(i_other)[0]
+        st = srv.stats()
+    assert st["refills"] >= 1
+    for g, pos in zip([long_] + late, res):
+        assert np.array_equal(pos, _alone(g))
+    assert np.array_equal(res_other, _alone(other, "double"))
