@@ -67,6 +67,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def calibration_copy(nbytes):
+    """One device-to-device copy of nbytes through the HIP runtime that
+    libfa2hip.so itself uses: a dispatch of known size for scaling the HBM
+    byte counters of a profiler run (FETCH_SIZE, WRITE_SIZE)."""
+    import ctypes
+    import forceatlas2_rc_amd  # noqa: F401  (loads the library and its runtime)
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), nbytes) or hip.hipMalloc(ctypes.byref(b), nbytes):
+        raise RuntimeError("could not allocate the calibration buffers")
+    if hip.hipMemcpy(b, a, nbytes, 3) or hip.hipDeviceSynchronize():  # device to device
+        raise RuntimeError("the calibration copy failed")
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"calibration copy of {nbytes} bytes done")
+
+
 def host_threads():
     """Threads for the CPU baseline: every core this process may run on,
     within the OpenMP thread count the host grants it (OMP_NUM_THREADS, when
@@ -452,6 +472,9 @@ def main():
     ap.add_argument("--serve-max-nodes", type=int, default=1_000_000)
     ap.add_argument("--serve-batches", type=int, default=4)
     ap.add_argument("--serve-chunk", type=int, default=25)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="first copy this many bytes device to device (a dispatch of known "
+                         "size for a profiler's HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU "
@@ -462,6 +485,8 @@ def main():
     from forceatlas2_rc_amd import Batch, workloads
     from forceatlas2_rc_amd.api import prepare_large
 
+    if args.copy_bytes:
+        calibration_copy(args.copy_bytes)
     t0 = time.time()
     n, e, pos = workloads.rmat_large(args.scale, args.edges, args.seed)
     log(f"graph n={n} m={len(e)} ({time.time() - t0:.1f} s)")
