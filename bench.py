@@ -447,6 +447,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--layout-iters", type=int, default=300,
+                    help="continue the timed layout to this many iterations, timed by chunks")
     ap.add_argument("--scale", type=int, default=21)
     ap.add_argument("--edges", type=int, default=16_000_000)
     ap.add_argument("--seed", type=int, default=2026)
@@ -525,10 +527,24 @@ def main():
     b.sync()
     dt = time.perf_counter() - t0
     pos_replay = b.positions(0)
-    b.close()
     ms = dt / args.steps * 1e3
     value = n * args.steps / dt
     log(f"timed: {args.steps} iterations, {ms:.4f} ms per iteration")
+    # the rest of the layout (to --layout-iters iterations in all), timed by
+    # chunks: how the cost of an iteration moves as the layout converges
+    progression, done_it = [], args.warmup + args.steps
+    while done_it < args.layout_iters:
+        k = min(25, args.layout_iters - done_it)
+        t1 = time.perf_counter()
+        b.run(k)
+        b.sync()
+        progression.append(dict(iterations=[done_it + 1, done_it + k],
+                                ms_per_iteration=(time.perf_counter() - t1) / k * 1e3))
+        done_it += k
+    b.close()
+    if progression:
+        log("layout progression (ms per iteration):",
+            [round(p_["ms_per_iteration"], 3) for p_ in progression])
 
     # the same iterations launched one kernel at a time with HIP events
     # between the phases: phase times and the traversal's duration
@@ -591,6 +607,7 @@ def main():
                            stage_ms={k: v * 1e3 for k, v in stages.items()},
                            profiled_ms_per_step=dt_prof / args.steps * 1e3,
                            replay_equals_eager=bool(np.array_equal(pos_replay, pos_out)),
+                           layout_progression=progression,
                            positions_finite=finite,
                            ingest=dict(host_numpy_prepare_s=t_prep, upload_prepared_s=t_up,
                                        device_prepare_with_upload_s=t_dev)),

@@ -1221,6 +1221,169 @@ k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restric
     }
 }
 
+// One step of a wave's walk (the body of k_repulse's walk, without the
+// counters) over the record `node`; returns the next record, -1 at the end.
+template <typename R, bool ANY_ADJUST>
+__device__ __forceinline__ int walk_step(const TNode<R> &node, typename Vec<R>::v4 me,
+                                         R theta2, bool adjust, int &done, ForceSum<R> &sum,
+                                         const typename Vec<R>::v4 *__restrict__ psort)
+{
+    using V4 = typename Vec<R>::v4;
+    const V4 rc = node.c;
+    const int4 rl = node.l;
+    asm volatile("" ::"s"(rc.x), "s"(rc.y), "s"(rc.z), "s"(rc.w), "s"(rl.x), "s"(rl.y),
+                 "s"(rl.z), "s"(rl.w));
+    const R xd = me.x - rc.x, yd = me.y - rc.y;
+    const R d2 = sumsq_rn(xd, yd);
+    const uint64_t am = __builtin_amdgcn_ballot_w64(rl.z > done);
+    const uint64_t fm = __builtin_amdgcn_ballot_w64(d2 * theta2 > rc.w);
+    if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {
+        asm volatile("");
+        sum.add(xd, yd, rc.z * frcp_r(d2));
+        done = rl.w;
+    }
+    const uint64_t tm = am & ~fm;
+    if (tm == 0) return rl.y;
+    if (rl.w - rl.z >= kLeafSize) return rl.x;
+    const int j0 = rl.z, j1 = rl.w;
+    V4 o[kLeafSize];
+#pragma unroll
+    for (int u = 0; u < kLeafSize; u++) o[u] = psort[j0 + u];
+    if (sizeof(R) == 4)
+        asm volatile("" ::"s"(o[0].x), "s"(o[0].y), "s"(o[0].z), "s"(o[1].x), "s"(o[1].y),
+                     "s"(o[1].z), "s"(o[2].x), "s"(o[2].y), "s"(o[2].z), "s"(o[3].x),
+                     "s"(o[3].y), "s"(o[3].z), "s"(o[4].x), "s"(o[4].y), "s"(o[4].z),
+                     "s"(o[5].x), "s"(o[5].y), "s"(o[5].z), "s"(o[6].x), "s"(o[6].y),
+                     "s"(o[6].z), "s"(o[7].x), "s"(o[7].y), "s"(o[7].z));
+    if (__builtin_amdgcn_inverse_ballot_w64(tm)) {
+#pragma unroll
+        for (int u = 0; u < kLeafSize; u++) {
+            if (j0 + u > j1) break;
+            if (ANY_ADJUST && adjust) {
+                lane_pair_adjust<R>(me, o[u], sum, true);
+            } else {
+                const R pxd = me.x - o[u].x, pyd = me.y - o[u].y;
+                const R pd2 = pxd * pxd + pyd * pyd;
+                R w = pd2 > (R) 0 ? o[u].z * frcp_r(pd2) : (R) 0;
+                if (sizeof(R) == 4) asm volatile("" : "+v"(w));
+                sum.add(pxd, pyd, w);
+            }
+        }
+        done = j1;
+    }
+    return rl.y;
+}
+
+// The force that leaves the traversal: repulsion minus gravity
+// (pipeline.cl:404-411), written in the original order.
+template <typename R>
+__device__ __forceinline__ void repulse_out(const GraphDesc &G, typename Vec<R>::v4 me,
+                                            const ForceSum<R> &sum, uint32_t dst,
+                                            double2 *__restrict__ F)
+{
+    const double sscale = G.scaling * (double) me.z;
+    double fx = sscale * sum.x, fy = sscale * sum.y;
+    const R grav = (R) G.gravity;
+    if (grav != (R) 0) {
+        const R dist = fsqrt_r(me.x * me.x + me.y * me.y);
+        if (dist > (R) 0) {
+            const R gg = (G.flags & GF_STRONG) ? grav * me.z : grav * me.z / dist;
+            fx -= (double) (me.x * gg);
+            fy -= (double) (me.y * gg);
+        }
+    }
+    F[dst] = make_double2(fx, fy);
+}
+
+// The traversal with two walks per wave: lanes hold node k of the first 64
+// sorted positions of the wave's 128 and node k + 64 of the second, and the
+// wave walks both unions in one loop. Each iteration issues the records of
+// both walks before either step, so one load latency serves two steps and
+// either walk's arithmetic covers the other's wait. Each lane's terms, and
+// their order, are those of k_repulse. A wave whose 128 nodes do not all
+// belong to one graph walks its two halves one after the other, graph by
+// graph, as k_repulse does.
+template <typename R, bool ANY_ADJUST, bool WIDE>
+__global__ void __launch_bounds__(1024)
+k_repulse2(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
+           const GraphState *__restrict__ gs, const typename Vec<R>::v4 *__restrict__ psort,
+           const TNode<R> *__restrict__ tn, const uint32_t *__restrict__ order,
+           double2 *__restrict__ F, int xrun)
+{
+    using V4 = typename Vec<R>::v4;
+    auto record = [&](int r) -> TNode<R> {
+        return WIDE ? tn[r]
+                    : *(const TNode<R> *) ((const char *) tn +
+                                           (uint32_t) r * (uint32_t) sizeof(TNode<R>));
+    };
+    const int lane = lane_id();
+    const int wv = xcd_block((int) blockIdx.x, (int) gridDim.x, xrun) * (int) (blockDim.x >> 6) +
+                   (int) (threadIdx.x >> 6);
+    const int kk0[2] = {wv * 128 + lane, wv * 128 + 64 + lane};
+    int kk[2], gg[2];
+    bool vv[2];
+    V4 me[2];
+    ForceSum<R> sum[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        kk[h] = kk0[h] < N ? kk0[h] : N - 1;
+        vv[h] = kk0[h] < N && !gs[gid[kk[h]]].retired;
+        gg[h] = vv[h] ? gid[kk[h]] : -1;
+        me[h] = psort[kk[h]];
+    }
+    const uint64_t any0 = __ballot(vv[0]), any1 = __ballot(vv[1]);
+    if ((any0 | any1) == 0) return;
+    const int lead = __ffsll((unsigned long long) (any0 ? any0 : any1)) - 1;
+    const int g = __builtin_amdgcn_readlane(any0 ? gg[0] : gg[1], lead);
+    const bool one = __ballot((vv[0] && gg[0] != g) || (vv[1] && gg[1] != g)) == 0;
+    if (one && any0 && any1) {
+        const GraphDesc G = gd[g];
+        const bool adjust = ANY_ADJUST && (G.flags & GF_ADJUST);
+        const R theta2 = (R) G.theta2;
+        int done0 = vv[0] ? INT_MIN + 1 : INT_MAX, done1 = vv[1] ? INT_MIN + 1 : INT_MAX;
+        int c0 = G.ioff, c1 = G.ioff;
+        // both walks, then the one that goes on alone (each is a
+        // depth-first pass that meets every node of the tree at most once,
+        // and ends at -1)
+        while (c0 >= 0 && c1 >= 0) {
+            const TNode<R> n0 = record(c0), n1 = record(c1);
+            c0 = __builtin_amdgcn_readfirstlane(
+                walk_step<R, ANY_ADJUST>(n0, me[0], theta2, adjust, done0, sum[0], psort));
+            c1 = __builtin_amdgcn_readfirstlane(
+                walk_step<R, ANY_ADJUST>(n1, me[1], theta2, adjust, done1, sum[1], psort));
+        }
+        while (c0 >= 0)
+            c0 = __builtin_amdgcn_readfirstlane(walk_step<R, ANY_ADJUST>(
+                record(c0), me[0], theta2, adjust, done0, sum[0], psort));
+        while (c1 >= 0)
+            c1 = __builtin_amdgcn_readfirstlane(walk_step<R, ANY_ADJUST>(
+                record(c1), me[1], theta2, adjust, done1, sum[1], psort));
+    } else {
+        // the halves one after the other, each graph by graph
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            uint64_t todo = __ballot(vv[h]);
+            while (todo) {
+                const int leader = __ffsll((unsigned long long) todo) - 1;
+                const int gh = __builtin_amdgcn_readlane(gg[h], leader);
+                const bool mine = gg[h] == gh;
+                todo &= ~__ballot(mine);
+                const GraphDesc G = gd[gh];
+                const bool adjust = ANY_ADJUST && (G.flags & GF_ADJUST);
+                const R theta2 = (R) G.theta2;
+                int done = mine ? INT_MIN + 1 : INT_MAX;
+                int c = G.ioff;
+                while (c >= 0)
+                    c = __builtin_amdgcn_readfirstlane(walk_step<R, ANY_ADJUST>(
+                        record(c), me[h], theta2, adjust, done, sum[h], psort));
+            }
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+        if (vv[h]) repulse_out<R>(gd[gg[h]], me[h], sum[h], order[kk[h]], F);
+}
+
 // ---------------------------------------------------------------------------
 // Attraction along the edges (pipeline.cl:392-429), added to the force that
 // the repulsion kernel left (repulsion minus gravity). Balanced by degree:
@@ -1730,11 +1893,16 @@ template <typename R> struct EngineImpl final : Engine {
         const bool wide = ((size_t) NI + (size_t) N) * sizeof(TNode<R>) >= ((size_t) 1 << 32);
         if (threads <= 0) threads = shape.repulse_threads;
         if (xrun < 0) xrun = shape.repulse_xcd_run;
-        const unsigned blocks = ceil_div(N, threads);
+        // the timed traversal walks two halves per wave (k_repulse2), the
+        // counting pass one (k_repulse): the same terms either way
+        const unsigned blocks = ceil_div(N, COUNT ? threads : 2 * threads);
 #define FA2_REPULSE(ADJ, WIDE)                                                                   \
-    k_repulse<R, ADJ, COUNT, WIDE><<<blocks, threads, 0, stream>>>(N, NI, d_gid.p, d_gd.p, d_gs.p, \
-                                                                   d_psort.p, d_tn.p, d_valA.p,    \
-                                                                   Fout, xrun, cnt)
+    if (COUNT)                                                                                   \
+        k_repulse<R, ADJ, COUNT, WIDE><<<blocks, threads, 0, stream>>>(                          \
+            N, NI, d_gid.p, d_gd.p, d_gs.p, d_psort.p, d_tn.p, d_valA.p, Fout, xrun, cnt);      \
+    else                                                                                         \
+        k_repulse2<R, ADJ, WIDE><<<blocks, threads, 0, stream>>>(                               \
+            N, NI, d_gid.p, d_gd.p, d_gs.p, d_psort.p, d_tn.p, d_valA.p, Fout, xrun)
         if (any_adjust) {
             if (wide) FA2_REPULSE(true, true);
             else FA2_REPULSE(true, false);
