@@ -396,11 +396,13 @@ def serving_leg(args):
     n_sub = min(len(reqs), args.serve_sub_requests)
     arrive_sub = arrive[:n_sub] * (args.serve_rate / args.serve_sub_rate)
     out_sub, span_sub, lat_sub, stats_sub = serve_stream(reqs[:n_sub], arrive_sub, args)
-    # one at a time, in arrival order
+    # one at a time, in arrival order, as fa2_full_run lays out one graph
+    # (default launch shapes: no autotuning, which the server does not do
+    # either)
     svc, same = [], 0
     for k, (g, prec, _) in enumerate(reqs):
         t1 = time.perf_counter()
-        with Batch([g], precision=prec) as b:
+        with Batch([g], precision=prec, autotune=False) as b:
             b.run(g["iterations"])
             b.sync()
             p = b.positions(0)
@@ -431,6 +433,7 @@ def serving_leg(args):
                 speedup=(len(reqs) / span) / (len(reqs) / svc.sum()),
                 size_class_ceiling=dict(speedup=ceiling, classes=classes),
                 batches=stats["batches"], refills=stats["refills"],
+                refill_s=stats["refill_s"],
                 max_batches_running=stats["peak_running"],
                 below_capacity=dict(rate=args.serve_sub_rate, requests=n_sub, span_s=span_sub,
                                     layouts_per_s=n_sub / span_sub,

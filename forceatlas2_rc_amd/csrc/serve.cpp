@@ -99,6 +99,7 @@ struct fa2_server {
     bool stop = false;
     // statistics
     long long batches = 0, admitted = 0, refills = 0;
+    double refill_s = 0; // worker time spent rebuilding batches
     int peak_running = 0;
 
     void loop();
@@ -202,6 +203,7 @@ bool fa2_server::refill(Running *r, const std::vector<int> &done)
         refills++;
     }
     r->bytes = keep_bytes + take_bytes;
+    const double t_start = now_s();
     // the remaining requests' state
     std::string err;
     for (size_t q = 0; q < keep.size() && err.empty(); q++) {
@@ -229,8 +231,9 @@ bool fa2_server::refill(Running *r, const std::vector<int> &done)
         if (r->e->run(r->target) != 0 || hipEventRecord(r->ev, r->st) != hipSuccess)
             err = fa2_gpu_error();
     }
+    std::lock_guard<std::mutex> lk(mu);
+    refill_s += now_s() - t_start;
     if (!err.empty()) {
-        std::lock_guard<std::mutex> lk(mu);
         for (Job *j : all) {
             j->state = kFailed;
             j->error = err;
@@ -577,7 +580,7 @@ extern "C" int fa2_serve_wait(fa2_server *s, int job, double *pos, double times[
     return 0;
 }
 
-extern "C" int fa2_serve_stats(fa2_server *s, double out[5])
+extern "C" int fa2_serve_stats(fa2_server *s, double out[6])
 {
     if (!s) return 1;
     std::lock_guard<std::mutex> lk(s->mu);
@@ -586,6 +589,7 @@ extern "C" int fa2_serve_stats(fa2_server *s, double out[5])
     out[2] = (double) s->peak_running;
     out[3] = (double) s->budget;
     out[4] = (double) s->refills;
+    out[5] = s->refill_s;
     return 0;
 }
 
