@@ -433,7 +433,7 @@ def serving_leg(args):
                 speedup=(len(reqs) / span) / (len(reqs) / svc.sum()),
                 size_class_ceiling=dict(speedup=ceiling, classes=classes),
                 batches=stats["batches"], refills=stats["refills"],
-                refill_s=stats["refill_s"],
+                refill_s=stats["refill_s"], uncaptured_batches=stats["uncaptured"],
                 max_batches_running=stats["peak_running"],
                 below_capacity=dict(rate=args.serve_sub_rate, requests=n_sub, span_s=span_sub,
                                     layouts_per_s=n_sub / span_sub,

@@ -614,10 +614,11 @@ class Server:
         return xy.reshape(2, n).T.copy(), dict(queued=t[0], running=t[1], total=t[2])
 
     def stats(self):
-        out = np.zeros(6)
+        out = np.zeros(7)
         lib.fa2_serve_stats(self.h, _ptr(out))
         return dict(batches=int(out[0]), admitted=int(out[1]), peak_running=int(out[2]),
-                    budget_bytes=out[3], refills=int(out[4]), refill_s=float(out[5]))
+                    budget_bytes=out[3], refills=int(out[4]), refill_s=float(out[5]),
+                    uncaptured=int(out[6]))
 
     def close(self):
         if self.h:

@@ -1767,6 +1767,16 @@ k_apply(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
 // Host side
 // ---------------------------------------------------------------------------
 
+// A copy in the order of stream st, waited for: it sees the work enqueued on
+// st before it and no other stream's (the null stream's synchronous copies
+// would also wait for other threads' blocking streams).
+static hipError_t copy_on(void *dst, const void *src, size_t bytes, hipMemcpyKind kind,
+                          hipStream_t st)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+    return e == hipSuccess ? hipStreamSynchronize(st) : e;
+}
+
 template <typename R> struct EngineImpl final : Engine {
     using V2 = typename Vec<R>::v2;
     using V4 = typename Vec<R>::v4;
@@ -2391,6 +2401,14 @@ template <typename R> bool EngineImpl<R>::capture_pair()
     enqueue_iteration(nullptr);
     hipGraph_t g = nullptr;
     if (hipStreamEndCapture(stream, &g) != hipSuccess || !g) {
+        // the stream leaves capture whatever the outcome; the iterations then
+        // run launch by launch
+        if (g) (void) hipGraphDestroy(g);
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+            hipGraph_t dead = nullptr;
+            if (hipStreamEndCapture(stream, &dead) == hipSuccess && dead) (void) hipGraphDestroy(dead);
+        }
         (void) hipGetLastError();
         return false;
     }
@@ -2475,7 +2493,7 @@ template <typename R> int EngineImpl<R>::sync()
     }
     ev_iters.clear();
     std::vector<GraphState> gs(G);
-    e = hipMemcpy(gs.data(), d_gs.p, G * sizeof(GraphState), hipMemcpyDeviceToHost);
+    e = copy_on(gs.data(), d_gs.p, G * sizeof(GraphState), hipMemcpyDeviceToHost, stream);
     if (e != hipSuccess) {
         set_hip_error("the computation on the GPU failed", e);
         return 1;
@@ -2492,7 +2510,8 @@ template <typename R> int EngineImpl<R>::graph_status(int graph)
 {
     if (graph < 0 || graph >= G) return -1;
     GraphState s;
-    if (hipMemcpy(&s, d_gs.p + graph, sizeof(s), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    if (copy_on(&s, d_gs.p + graph, sizeof(s), hipMemcpyDeviceToHost, stream) != hipSuccess)
+        return -1;
     return s.status;
 }
 
@@ -2564,7 +2583,7 @@ template <typename R> int EngineImpl<R>::speed_state(int graph, double out[4])
         return 1;
     }
     GraphState s;
-    hipError_t e = hipMemcpy(&s, d_gs.p + graph, sizeof(s), hipMemcpyDeviceToHost);
+    hipError_t e = copy_on(&s, d_gs.p + graph, sizeof(s), hipMemcpyDeviceToHost, stream);
     if (e != hipSuccess) {
         set_hip_error("could not fetch the speed from the device", e);
         return 1;
@@ -2593,7 +2612,7 @@ int EngineImpl<R>::tree(int graph, uint32_t *keys_sorted, uint32_t *order, int *
     std::vector<int> pl(n), pi(n - 1);
     hipError_t e = hipStreamSynchronize(stream);
     auto down = [&](void *dst, const void *src, size_t bytes) {
-        if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && bytes) e = copy_on(dst, src, bytes, hipMemcpyDeviceToHost, stream);
     };
     down(ks.data(), d_keyA.p + D.noff, n * sizeof(uint32_t));
     down(ord.data(), d_valA.p + D.noff, n * sizeof(uint32_t));
@@ -2639,8 +2658,8 @@ template <typename R> int EngineImpl<R>::regions(int graph, double *rec, int *sk
     std::vector<TNode<R>> tn(n - 1);
     hipError_t e = hipStreamSynchronize(stream);
     if (e == hipSuccess)
-        e = hipMemcpy(tn.data(), d_tn.p + D.ioff, (n - 1) * sizeof(TNode<R>),
-                      hipMemcpyDeviceToHost);
+        e = copy_on(tn.data(), d_tn.p + D.ioff, (n - 1) * sizeof(TNode<R>), hipMemcpyDeviceToHost,
+                    stream);
     if (e != hipSuccess) {
         set_hip_error("could not fetch the tree from the device", e);
         return 1;
@@ -2674,7 +2693,7 @@ int EngineImpl<R>::csr(int graph, double *mass, int *inc_off, int *inc_node, dou
     std::vector<R> m(n);
     hipError_t e = hipStreamSynchronize(stream);
     auto down = [&](void *dst, const void *src, size_t bytes) {
-        if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && bytes) e = copy_on(dst, src, bytes, hipMemcpyDeviceToHost, stream);
     };
     down(off.data(), d_inc_off.p + D.noff, (n + 1) * sizeof(int));
     down(m.data(), d_mass.p + D.noff, n * sizeof(R));
@@ -2983,18 +3002,21 @@ int EngineImpl<R>::set_state(int graph, const double *pos, const double *fold, d
     }
     GraphState st;
     hipError_t e = hipStreamSynchronize(stream);
-    if (e == hipSuccess) e = hipMemcpy(&st, d_gs.p + graph, sizeof(st), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = copy_on(&st, d_gs.p + graph, sizeof(st), hipMemcpyDeviceToHost, stream);
     st.speed = speed;
     st.eff = eff;
     st.status = 0;
-    if (e == hipSuccess) e = hipMemcpy(d_gs.p + graph, &st, sizeof(st), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_pos.p + D.noff, p.data(), n * sizeof(V2), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = copy_on(d_gs.p + graph, &st, sizeof(st), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+        e = copy_on(d_pos.p + D.noff, p.data(), n * sizeof(V2), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess && Acc<R>::separate)
-        e = hipMemcpy(d_acc.p + D.noff, a.data(), n * sizeof(double2), hipMemcpyHostToDevice);
+        e = copy_on(d_acc.p + D.noff, a.data(), n * sizeof(double2), hipMemcpyHostToDevice, stream);
     // the next iteration reads the previous force from F[1 - parity]
     if (e == hipSuccess)
-        e = hipMemcpy(d_F[1 - parity].p + D.noff, fo.data(), n * sizeof(double2),
-                      hipMemcpyHostToDevice);
+        e = copy_on(d_F[1 - parity].p + D.noff, fo.data(), n * sizeof(double2),
+                    hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) {
         k_tile_bbox<R><<<T, 256, 0, stream>>>(d_tiles.p, d_pos.p, d_tbox.p);
         k_pack_node4<R><<<ceil_div(N, 256), 256, 0, stream>>>(N, d_pos.p, d_mass.p, d_size.p,

@@ -4,22 +4,24 @@
 //
 // Requests are copied in by fa2_serve_submit (prepared incidence lists) or
 // fa2_serve_submit_edges (edge lists, prepared on the device) and wait in
-// arrival order. A worker thread admits them by device-memory footprint:
-// when a stream is free, the request at the head of the queue and every
-// later one of the same precision and kind that fits in what the running
-// batches leave of the budget join one new batch (engine.hip) on that
-// stream.
+// arrival order. Each stream of the server has a host thread of its own
+// (a lane) that admits them by device-memory footprint: when the lane is
+// free, the request at the head of the queue and every later one of the
+// same precision and kind that fits in what the running batches leave of
+// the budget join one new batch (engine.hip) on the lane's stream.
 // Batches on different streams run at the same time, so requests that
-// arrive while others are on the GPU start as soon as a stream and memory
-// are free, and their work overlaps on the device. Each batch advances in
-// chunks of iterations; after each chunk the worker reads how far every
-// graph has come, fetches the positions of the graphs that reached their
-// iteration count, and retires them (their repulsion and attraction are
-// skipped from then on). A batch's memory, allocated in its stream's order,
-// is released when its last request is done.
+// arrive while others are on the GPU start as soon as a lane and memory
+// are free, and their work overlaps on the device; a lane busy on the host
+// (building a batch, reading results) holds up only its own stream. Each
+// batch advances in chunks of iterations; after each chunk its lane reads
+// how far every graph has come, fetches the positions of the graphs that
+// reached their iteration count, and retires them (their repulsion and
+// attraction are skipped from then on). A batch's memory, allocated in its
+// stream's order, is released when its last request is done.
 //
-// Continuous batching: when every stream is taken and requests wait, a
-// batch that has finished some of its requests takes waiting ones in at its
+// Continuous batching: when no lane can take the waiting requests (every
+// lane busy, or the memory taken), a batch that has finished some of its
+// requests takes waiting ones in at its
 // next chunk boundary. Its remaining requests' state (positions, previous
 // force, speed and speed efficiency, iterations done) is read back, the
 // batch is rebuilt on its stream with them and the newcomers that fit in
@@ -94,16 +96,19 @@ struct fa2_server {
     std::deque<Job *> queue;
     std::vector<std::unique_ptr<Job>> jobs;
     std::vector<Running *> running;
-    std::vector<hipStream_t> streams, free_streams;
-    std::thread worker;
+    std::vector<hipStream_t> streams;
+    std::vector<std::thread> lanes;
+    int idle = 0; // lanes without a batch
     bool stop = false;
     // statistics
     long long batches = 0, admitted = 0, refills = 0;
-    double refill_s = 0; // worker time spent rebuilding batches
+    long long uncaptured = 0; // batches whose iterations ran launch by launch
+    double refill_s = 0; // lane time spent rebuilding batches
     int peak_running = 0;
 
-    void loop();
-    bool admit(std::unique_lock<std::mutex> &lk);
+    void lane(hipStream_t st);
+    bool can_admit() const;
+    Running *admit(std::unique_lock<std::mutex> &lk, hipStream_t st, hipEvent_t ev);
     bool advance(Running *r);
     void finish_batch(Running *r);
     std::string open_engine(Running *r, const std::vector<Job *> &jobs, int use_double,
@@ -134,7 +139,7 @@ std::string fa2_server::open_engine(Running *r, const std::vector<Job *> &jobs, 
     // batches already run side by side: one stream each (a second one per
     // batch would share the hardware queues between batches), and the
     // default launch shapes (the autotuner synchronises the device and
-    // would hold the worker between chunks)
+    // would hold the lane between chunks)
     r->e->overlap = 0;
     if (r->e->open(gr.data(), (int) G) != 0) return fa2_gpu_error();
     for (size_t k = 0; k < G; k++) {
@@ -149,8 +154,8 @@ std::string fa2_server::open_engine(Running *r, const std::vector<Job *> &jobs, 
     return std::string();
 }
 
-// At a chunk boundary of r, when no stream is free and requests of r's
-// precision and kind wait: those that fit in the budget, with what r's
+// At a chunk boundary of r, when no lane can admit the waiting requests
+// and some of r's precision and kind wait: those that fit in the budget, with what r's
 // finished requests give back, join r's remaining ones in a rebuilt batch
 // on r's stream. A rebuild reads back and uploads again the remaining
 // requests, so it is made only when some of them finished or when the
@@ -177,7 +182,7 @@ bool fa2_server::refill(Running *r, const std::vector<int> &done)
     size_t take_bytes = 0;
     {
         std::lock_guard<std::mutex> lk(mu);
-        if (queue.empty() || !free_streams.empty()) return false;
+        if (queue.empty() || can_admit()) return false;
         size_t avail = budget - (used - r->bytes) - keep_bytes;
         long long take_nodes = 0;
         std::deque<Job *> rest;
@@ -233,6 +238,7 @@ bool fa2_server::refill(Running *r, const std::vector<int> &done)
     }
     std::lock_guard<std::mutex> lk(mu);
     refill_s += now_s() - t_start;
+    if (err.empty() && !r->e->use_graph) uncaptured++;
     if (!err.empty()) {
         for (Job *j : all) {
             j->state = kFailed;
@@ -246,10 +252,21 @@ bool fa2_server::refill(Running *r, const std::vector<int> &done)
     return true;
 }
 
-// One new batch from the requests at the head of the queue that fit.
-bool fa2_server::admit(std::unique_lock<std::mutex> &lk)
+// Whether an idle lane may start a batch now: a request waits and the one
+// at the head fits in what the running batches leave of the budget (or can
+// never fit, and is to be failed). Called with the lock held.
+bool fa2_server::can_admit() const
 {
-    if (queue.empty() || free_streams.empty()) return false;
+    if (queue.empty() || idle == 0) return false;
+    const size_t b = queue.front()->bytes;
+    return b > budget || used + b <= budget;
+}
+
+// One new batch on stream st from the requests at the head of the queue
+// that fit; nullptr when none could start. Called with the lock held, which
+// is released while the batch is built and held again on return.
+Running *fa2_server::admit(std::unique_lock<std::mutex> &lk, hipStream_t st, hipEvent_t ev)
+{
     // a request that alone exceeds the budget can never run
     while (!queue.empty() && queue.front()->bytes > budget) {
         Job *j = queue.front();
@@ -259,7 +276,7 @@ bool fa2_server::admit(std::unique_lock<std::mutex> &lk)
         j->t_done = now_s();
         cv_done.notify_all();
     }
-    if (queue.empty() || used + queue.front()->bytes > budget) return false;
+    if (queue.empty() || used + queue.front()->bytes > budget) return nullptr;
     const int use_double = queue.front()->p.use_double;
     const bool has_edges = queue.front()->has_edges;
     std::vector<Job *> take;
@@ -275,20 +292,20 @@ bool fa2_server::admit(std::unique_lock<std::mutex> &lk)
         take.push_back(j);
         it = queue.erase(it);
     }
-    hipStream_t st = free_streams.back();
-    free_streams.pop_back();
     used += bytes;
+    const double t = now_s();
+    for (Job *j : take) {
+        j->t_admit = t;
+        j->state = kRunning;
+    }
     lk.unlock();
 
     // build the batch without holding the lock: the upload takes a while
     Running *r = new Running();
     r->st = st;
+    r->ev = ev;
     r->bytes = bytes;
-    const double t = now_s();
-    for (Job *j : take) j->t_admit = t;
     std::string err = open_engine(r, take, use_double, has_edges);
-    if (err.empty() && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
-        err = "could not create a HIP event";
     if (err.empty()) {
         r->target = std::min(chunk, r->longest);
         if (r->e->run(r->target) != 0 || hipEventRecord(r->ev, st) != hipSuccess)
@@ -302,20 +319,20 @@ bool fa2_server::admit(std::unique_lock<std::mutex> &lk)
             j->error = err;
             j->t_done = now_s();
         }
-        if (r->ev) (void) hipEventDestroy(r->ev);
-        delete r->e;
+        lk.unlock();
+        delete r->e; // waits for its stream, frees in the stream's order
+        lk.lock();
         delete r;
         used -= bytes;
-        free_streams.push_back(st);
         cv_done.notify_all();
-        return true;
+        return nullptr;
     }
-    for (Job *j : take) j->state = kRunning;
     running.push_back(r);
     batches++;
+    if (!r->e->use_graph) uncaptured++;
     admitted += (long long) take.size();
     peak_running = std::max(peak_running, (int) running.size());
-    return true;
+    return r;
 }
 
 // After a completed chunk: hand out the finished requests, enqueue the next
@@ -392,38 +409,56 @@ bool fa2_server::advance(Running *r)
 void fa2_server::finish_batch(Running *r)
 {
     delete r->e; // waits for its stream, frees in the stream's order
-    (void) hipEventDestroy(r->ev);
     std::lock_guard<std::mutex> lk(mu);
     used -= r->bytes;
-    free_streams.push_back(r->st);
     running.erase(std::find(running.begin(), running.end(), r));
     delete r;
 }
 
-void fa2_server::loop()
+// The host thread of stream st: start a batch when requests wait and fit,
+// run it chunk by chunk to its end, and again, until the server closes and
+// the queue is empty.
+void fa2_server::lane(hipStream_t st)
 {
     (void) hipSetDevice(device);
+    // this thread's calls leave the other lanes' graph captures alone
+    hipStreamCaptureMode mode = hipStreamCaptureModeRelaxed;
+    (void) hipThreadExchangeStreamCaptureMode(&mode);
+    hipEvent_t ev = nullptr;
+    // a blocking event: the lane sleeps while its chunk runs
+    const bool have_ev =
+        hipEventCreateWithFlags(&ev, hipEventDisableTiming | hipEventBlockingSync) == hipSuccess;
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
-        if (stop && queue.empty() && running.empty()) break;
-        bool busy = false;
-        while (admit(lk)) busy = true;
-        std::vector<Running *> ready;
-        for (Running *r : running)
-            if (hipEventQuery(r->ev) != hipErrorNotReady) ready.push_back(r);
-        lk.unlock();
-        for (Running *r : ready) {
-            busy = true;
-            if (advance(r)) finish_batch(r);
+        cv_work.wait(lk, [&] { return (stop && queue.empty()) || can_admit(); });
+        if (stop && queue.empty()) break;
+        if (!have_ev) { // nothing can run on this lane
+            Job *j = queue.front();
+            queue.pop_front();
+            j->state = kFailed;
+            j->error = "could not create a HIP event";
+            j->t_done = now_s();
+            cv_done.notify_all();
+            continue;
         }
-        lk.lock();
-        if (!busy) {
-            if (running.empty())
-                cv_work.wait(lk, [&] { return stop || !queue.empty(); });
-            else // a chunk takes at least tens of microseconds
-                cv_work.wait_for(lk, std::chrono::microseconds(50));
+        idle--;
+        Running *r = admit(lk, st, ev);
+        if (r) {
+            lk.unlock();
+            bool over = false;
+            while (!over) {
+                (void) hipEventSynchronize(r->ev); // errors surface in progress()
+                over = advance(r);
+            }
+            finish_batch(r);
+            lk.lock();
         }
+        idle++;
+        // memory came back: another lane may start the next batch
+        cv_work.notify_all();
     }
+    lk.unlock();
+    if (have_ev) (void) hipEventDestroy(ev);
 }
 
 extern "C" fa2_server *fa2_serve_open(int device, double budget_bytes, int max_batches,
@@ -467,8 +502,8 @@ extern "C" fa2_server *fa2_serve_open(int device, double budget_bytes, int max_b
         }
         s->streams.push_back(st);
     }
-    s->free_streams = s->streams;
-    s->worker = std::thread([s] { s->loop(); });
+    s->idle = max_batches;
+    for (hipStream_t st : s->streams) s->lanes.emplace_back([s, st] { s->lane(st); });
     return s;
 }
 
@@ -580,7 +615,7 @@ extern "C" int fa2_serve_wait(fa2_server *s, int job, double *pos, double times[
     return 0;
 }
 
-extern "C" int fa2_serve_stats(fa2_server *s, double out[6])
+extern "C" int fa2_serve_stats(fa2_server *s, double out[7])
 {
     if (!s) return 1;
     std::lock_guard<std::mutex> lk(s->mu);
@@ -590,6 +625,7 @@ extern "C" int fa2_serve_stats(fa2_server *s, double out[6])
     out[3] = (double) s->budget;
     out[4] = (double) s->refills;
     out[5] = s->refill_s;
+    out[6] = (double) s->uncaptured;
     return 0;
 }
 
@@ -601,7 +637,8 @@ extern "C" void fa2_serve_close(fa2_server *s)
         s->stop = true;
         s->cv_work.notify_all();
     }
-    if (s->worker.joinable()) s->worker.join();
+    for (std::thread &t : s->lanes)
+        if (t.joinable()) t.join();
     for (hipStream_t st : s->streams) (void) hipStreamDestroy(st);
     delete s;
 }

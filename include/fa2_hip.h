@@ -266,8 +266,9 @@ int fa2_serve_wait(fa2_server *s, int job, double *pos, double times[3]);
 
 /* out = (batches formed, requests admitted, most batches running at once,
  * memory budget in bytes, batches rebuilt to take waiting requests in,
- * seconds the server spent rebuilding them). */
-int fa2_serve_stats(fa2_server *s, double out[6]);
+ * seconds the server spent rebuilding them, batches formed whose iterations
+ * could not be captured into a HIP graph and ran launch by launch). */
+int fa2_serve_stats(fa2_server *s, double out[7]);
 
 /* Finish the queued and running requests, then release the server. */
 void fa2_serve_close(fa2_server *s);
