@@ -475,7 +475,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-sub-rate", type=float, default=3.0)
     ap.add_argument("--serve-max-nodes", type=int, default=1_000_000)
-    ap.add_argument("--serve-batches", type=int, default=4)
+    ap.add_argument("--serve-batches", type=int, default=8)
     ap.add_argument("--serve-chunk", type=int, default=25)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first copy this many bytes device to device (a dispatch of known "

@@ -573,7 +573,7 @@ class Server:
     one precision.
     """
 
-    def __init__(self, device=0, budget_bytes=0.0, max_batches=4, chunk=25):
+    def __init__(self, device=0, budget_bytes=0.0, max_batches=8, chunk=25):
         self.h = lib.fa2_serve_open(int(device), float(budget_bytes), int(max_batches),
                                     int(chunk))
         if not self.h:

@@ -19,6 +19,11 @@ struct EngineGraph {
     // the edge list, when the mass and incidence lists are to be built on
     // the device (p's mass and inc_* are then not read); else null
     const fa2_edges *edges = nullptr;
+    // the state a graph resumes from (as left by earlier iterations, see
+    // read_states): its previous force (x then y; null for zero, the start
+    // of a layout), speed and speed efficiency
+    const double *fold = nullptr;
+    double speed = 1.0, eff = 1.0;
 };
 
 struct Engine {
@@ -70,6 +75,13 @@ struct Engine {
     // (x then y, NULL for zero), speed and speed efficiency.
     virtual int set_state(int graph, const double *pos, const double *fold, double speed,
                           double eff) = 0;
+    // The state of `count` graphs at once, after the work enqueued before the
+    // call, with one transfer per array: positions pos[k] (x then y), last
+    // force fold[k] (x then y) and (total swinging, total traction, speed,
+    // speed efficiency) sp[k] of graph graphs[k]. Any of pos, fold, sp, or
+    // any of their entries, may be null.
+    virtual int read_states(int count, const int *graphs, double *const *pos,
+                            double *const *fold, double (*sp)[4]) = 0;
     // Launch shapes (fa2_tuning of fa2_hip.h); tune() times the candidates
     // on the current positions and keeps the fastest.
     virtual int tune(int force, fa2_tuning *chosen, double *times) = 0;

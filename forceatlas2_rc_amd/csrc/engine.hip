@@ -27,8 +27,11 @@
 // to run, and a graph gives the same result alone and in a batch as long
 // as the tiles of its nodes do not change (they depend only on the graph).
 #include <algorithm>
+#include <atomic>
 #include <map>
+#include <memory>
 #include <mutex>
+#include <thread>
 #include <tuple>
 #include <climits>
 #include <cmath>
@@ -1767,6 +1770,38 @@ k_apply(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
 // Host side
 // ---------------------------------------------------------------------------
 
+// Host work of EngineImpl::open split over threads: f(k) for every k in
+// [0, count), pieces taken in turn by up to `threads` threads (the calling
+// one among them; fewer when the system gives no more).
+constexpr int kHostPiece = 1 << 16;
+
+template <typename F> static void host_parallel(int count, int threads, const F &f)
+{
+    threads = std::max(1, std::min(threads, count));
+    std::atomic<int> next{0};
+    auto work = [&] {
+        for (int k = next.fetch_add(1); k < count; k = next.fetch_add(1)) f(k);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; t++) {
+        try {
+            pool.emplace_back(work);
+        } catch (...) {
+            break;
+        }
+    }
+    work();
+    for (std::thread &th : pool) th.join();
+}
+
+// Threads for host work of about `items` elements: one below a million.
+static int host_threads_for(long long items)
+{
+    if (items < (1LL << 20)) return 1;
+    const unsigned hc = std::thread::hardware_concurrency();
+    return (int) std::min(16u, std::max(1u, hc));
+}
+
 // A copy in the order of stream st, waited for: it sees the work enqueued on
 // st before it and no other stream's (the null stream's synchronous copies
 // would also wait for other threads' blocking streams).
@@ -1868,6 +1903,8 @@ template <typename R> struct EngineImpl final : Engine {
     int graph_status(int graph) override;
     bool capture_pair();
     int count_work(unsigned long long out[4]) override;
+    int read_states(int count, const int *graphs, double *const *pos, double *const *fold,
+                    double (*sp)[4]) override;
     int set_state(int graph, const double *pos, const double *fold, double speed,
                   double eff) override;
 
@@ -2005,20 +2042,56 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         level_cnt[L] = (int) h_units.size() - level_off[L];
     }
 
-    // packed host arrays
+    // packed host arrays, filled by pieces of at most kHostPiece nodes of one
+    // graph on several host threads; the large ones are left uninitialised
+    // (every element is written below)
     const size_t nn = (size_t) N;
-    std::vector<int> gid(nn), gi(NI > 0 ? NI : 1), inc_off(nn + 1);
-    std::vector<int> inc_node(from_edges ? 1 : (I > 0 ? I : 1));
-    std::vector<R> coef(from_edges ? 1 : (I > 0 ? I : 1)), mass(from_edges ? 1 : nn), size(nn);
-    std::vector<V2> pos(nn);
-    std::vector<double2> acc(Acc<R>::separate ? nn : 1);
-    std::vector<uint8_t> fixed(nn);
+    const size_t ni_all = from_edges ? 1 : (size_t) (I > 0 ? I : 1);
+    std::vector<int> inc_off(nn + 1);
+    std::unique_ptr<int[]> gid(new (std::nothrow) int[nn]);
+    std::unique_ptr<int[]> gi(new (std::nothrow) int[NI > 0 ? NI : 1]);
+    std::unique_ptr<int[]> inc_node(new (std::nothrow) int[ni_all]);
+    std::unique_ptr<R[]> coef(new (std::nothrow) R[ni_all]);
+    std::unique_ptr<R[]> mass(new (std::nothrow) R[from_edges ? 1 : nn]);
+    std::unique_ptr<R[]> size(new (std::nothrow) R[nn]);
+    std::unique_ptr<V2[]> pos(new (std::nothrow) V2[nn]);
+    std::unique_ptr<V4[]> node4(new (std::nothrow) V4[from_edges ? 1 : nn]);
+    std::unique_ptr<double2[]> acc(new (std::nothrow) double2[Acc<R>::separate ? nn : 1]);
+    std::unique_ptr<uint8_t[]> fixed(new (std::nothrow) uint8_t[nn]);
+    if (!gid || !gi || !inc_node || !coef || !mass || !size || !pos || !node4 || !acc || !fixed) {
+        set_message("out of host memory");
+        return 1;
+    }
+    struct Piece {
+        int g, i0, i1;          // graph, its nodes [i0, i1)
+        int bad = 0;            // 1: offsets out of order, 2: a node out of range
+        bool nonuniform = false;
+        std::vector<int4> cls[2][4], items, hubs; // attraction work, see below
+        int nslot = 0;
+    };
+    std::vector<Piece> pieces;
+    std::vector<long long> gbase(G); // each graph's first incidence
     long long ioff = 0;
     for (int g = 0; g < G; g++) {
+        gbase[g] = ioff;
+        if (!from_edges) ioff += gr[g].p->inc_off[gr[g].p->n];
+        for (int i = 0; i < h_gd[g].n; i += kHostPiece) {
+            Piece pc;
+            pc.g = g;
+            pc.i0 = i;
+            pc.i1 = std::min(h_gd[g].n, i + kHostPiece);
+            pieces.push_back(std::move(pc));
+        }
+    }
+    const int nthreads = host_threads_for((long long) N + (from_edges ? 0 : I));
+    host_parallel((int) pieces.size(), nthreads, [&](int k) {
+        Piece &pc = pieces[k];
+        const int g = pc.g;
         const fa2_full_params *p = gr[g].p;
         const GraphDesc &D = h_gd[g];
         const double *x = gr[g].pos, *y = gr[g].pos + p->n;
-        for (int i = 0; i < p->n; i++) {
+        const long long ib = gbase[g];
+        for (int i = pc.i0; i < pc.i1; i++) {
             const size_t q = (size_t) D.noff + i;
             gid[q] = g;
             size[q] = (p->adjust && p->size) ? (R) p->size[i] : (R) 0;
@@ -2027,35 +2100,48 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
             if (Acc<R>::separate) acc[q] = make_double2(x[i], y[i]);
             if (!from_edges) {
                 mass[q] = (R) p->mass[i];
-                inc_off[q] = (int) (ioff + p->inc_off[i]);
+                inc_off[q] = (int) (ib + p->inc_off[i]);
+                node4[q] = mk4<R>(pos[q].x, pos[q].y, mass[q], size[q]);
             }
         }
-        for (int i = 0; i < D.n - 1; i++) gi[(size_t) D.ioff + i] = g;
-        if (from_edges) continue;
+        for (int i = pc.i0; i < std::min(pc.i1, D.n - 1); i++) gi[(size_t) D.ioff + i] = g;
+        if (from_edges) return;
         const int ni = p->inc_off[p->n];
         bool uniform = true; // every coefficient of a node equal to its first, bit for bit
-        for (int i = 0; i < p->n; i++) {
+        for (int i = pc.i0; i < pc.i1; i++) {
             const int e0 = p->inc_off[i], e1 = p->inc_off[i + 1];
             if (e0 < 0 || e1 < e0 || e1 > ni) {
-                set_message("the incidence offsets must rise from 0 to the number of incidences");
-                return 1;
+                pc.bad = 1;
+                return;
             }
             const R first = e1 > e0 ? (R) p->inc_coef[e0] : (R) 0;
             for (int e = e0; e < e1; e++) {
                 const int j = p->inc_node[e];
                 if (j < 0 || j >= p->n) {
-                    set_message("edge refers to a node that does not exist");
-                    return 1;
+                    pc.bad = 2;
+                    return;
                 }
-                inc_node[(size_t) ioff + e] = D.noff + j;
+                inc_node[(size_t) ib + e] = D.noff + j;
                 const R c = (R) p->inc_coef[e];
-                coef[(size_t) ioff + e] = c;
+                coef[(size_t) ib + e] = c;
                 uniform = uniform && memcmp(&c, &first, sizeof(R)) == 0;
             }
         }
-        if (uniform) h_gd[g].flags |= GF_UNIFORM;
-        ioff += ni;
+        pc.nonuniform = !uniform;
+    });
+    std::vector<char> nonuniform(G, 0);
+    for (const Piece &pc : pieces) {
+        if (pc.bad) {
+            set_message(pc.bad == 1
+                            ? "the incidence offsets must rise from 0 to the number of incidences"
+                            : "edge refers to a node that does not exist");
+            return 1;
+        }
+        nonuniform[pc.g] |= pc.nonuniform;
     }
+    if (!from_edges)
+        for (int g = 0; g < G; g++)
+            if (!nonuniform[g]) h_gd[g].flags |= GF_UNIFORM;
     inc_off[nn] = (int) ioff;
     const hipStream_t ast = async_alloc ? stream : nullptr;
     if (from_edges) {
@@ -2086,40 +2172,72 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         I = inc_off[nn];
     }
     // attraction work: one lane per node up to 16 incidences, 16 lanes up
-    // to 256, then waves over chunks of kChunk incidences
-    for (int i = 0; i < N; i++) {
-        const int e0 = inc_off[i], e1 = inc_off[i + 1], deg = e1 - e0;
-        if (deg == 0) continue;
-        if (deg <= 16) {
-            h_class[0].push_back(make_int4(i, e0, e1, gid[i]));
-        } else if (deg <= 256) {
-            h_class[1].push_back(make_int4(i, e0, e1, gid[i]));
-        } else if (deg <= kChunk) {
-            h_items.push_back(make_int4(i, e0, e1, -1));
-        } else {
-            const int first = nslots;
-            for (int e = e0; e < e1; e += kChunk)
-                h_items.push_back(make_int4(i, e, std::min(e1, e + kChunk), nslots++));
-            h_hubs.push_back(make_int4(i, first, nslots - first, 0));
+    // to 256, then waves over chunks of kChunk incidences. Lanes of one wave
+    // run as many rounds as its longest list: each of the first two classes
+    // is ordered by degree band (4 bands, longest first), node order within
+    // a band. Every piece lists its own nodes; the lists are joined in piece
+    // order, which is node order.
+    host_parallel((int) pieces.size(), nthreads, [&](int k) {
+        Piece &pc = pieces[k];
+        const int noff = h_gd[pc.g].noff;
+        for (int i = noff + pc.i0; i < noff + pc.i1; i++) {
+            const int e0 = inc_off[i], e1 = inc_off[i + 1], deg = e1 - e0;
+            if (deg == 0) continue;
+            if (deg <= 16) {
+                pc.cls[0][3 - (deg - 1) / 4].push_back(make_int4(i, e0, e1, pc.g));
+            } else if (deg <= 256) {
+                pc.cls[1][3 - (deg - 1) / 64].push_back(make_int4(i, e0, e1, pc.g));
+            } else if (deg <= kChunk) {
+                pc.items.push_back(make_int4(i, e0, e1, -1));
+            } else { // hub partial sums: slots numbered within the piece
+                const int first = pc.nslot;
+                for (int e = e0; e < e1; e += kChunk)
+                    pc.items.push_back(make_int4(i, e, std::min(e1, e + kChunk), pc.nslot++));
+                pc.hubs.push_back(make_int4(i, first, pc.nslot - first, 0));
+            }
         }
+    });
+    // where each piece's lists go
+    const size_t P = pieces.size();
+    std::vector<size_t> at_cls(2 * 4 * P), at_items(P), at_hubs(P);
+    std::vector<int> at_slot(P);
+    size_t ncls[2] = {0, 0}, nitems = 0, nhubs = 0;
+    nslots = 0;
+    for (int c = 0; c < 2; c++)
+        for (int b = 0; b < 4; b++)
+            for (size_t k = 0; k < P; k++) {
+                at_cls[(c * 4 + b) * P + k] = ncls[c];
+                ncls[c] += pieces[k].cls[c][b].size();
+            }
+    for (size_t k = 0; k < P; k++) {
+        at_items[k] = nitems;
+        at_hubs[k] = nhubs;
+        at_slot[k] = nslots;
+        nitems += pieces[k].items.size();
+        nhubs += pieces[k].hubs.size();
+        nslots += pieces[k].nslot;
     }
-    // lanes of one wave run as many rounds as its longest list: order each
-    // class by degree band (4 bands, longest first), node order within a band
-    auto by_band = [](std::vector<int4> &v, int width) {
-        auto slot = [width](const int4 &a) { return 3 - (a.z - a.y - 1) / width; };
-        size_t at[4] = {0, 0, 0, 0};
-        for (const int4 &a : v) at[slot(a)]++;
-        for (size_t k = 0, sum = 0; k < 4; k++) {
-            const size_t c = at[k];
-            at[k] = sum;
-            sum += c;
+    for (int c = 0; c < 2; c++) h_class[c].resize(ncls[c]);
+    h_items.resize(nitems);
+    h_hubs.resize(nhubs);
+    host_parallel((int) P, nthreads, [&](int k) {
+        Piece &pc = pieces[k];
+        for (int c = 0; c < 2; c++)
+            for (int b = 0; b < 4; b++)
+                std::copy(pc.cls[c][b].begin(), pc.cls[c][b].end(),
+                          h_class[c].begin() + at_cls[(c * 4 + b) * P + k]);
+        const int sb = at_slot[k];
+        for (size_t q = 0; q < pc.items.size(); q++) {
+            int4 it = pc.items[q];
+            if (it.w >= 0) it.w += sb;
+            h_items[at_items[k] + q] = it;
         }
-        std::vector<int4> out(v.size());
-        for (const int4 &a : v) out[at[slot(a)]++] = a;
-        v.swap(out);
-    };
-    by_band(h_class[0], 4);
-    by_band(h_class[1], 64);
+        for (size_t q = 0; q < pc.hubs.size(); q++) {
+            int4 h = pc.hubs[q];
+            h.y += sb;
+            h_hubs[at_hubs[k] + q] = h;
+        }
+    });
 
     HipStatus hs;
     auto up = [&](void *dst, const void *src, size_t bytes) {
@@ -2176,22 +2294,24 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         return 1;
     }
     std::vector<GraphState> gs(G);
-    for (auto &s : gs) {
-        memset(&s, 0, sizeof(s));
-        s.speed = 1.0;
-        s.eff = 1.0;
+    bool any_fold = false;
+    for (int g = 0; g < G; g++) {
+        memset(&gs[g], 0, sizeof(GraphState));
+        gs[g].speed = gr[g].speed;
+        gs[g].eff = gr[g].eff;
+        any_fold = any_fold || gr[g].fold;
     }
     up(d_gd.p, h_gd.data(), G * sizeof(GraphDesc));
     up(d_gs.p, gs.data(), G * sizeof(GraphState));
     up(d_tiles.p, h_tiles.data(), T * sizeof(Tile));
     up(d_units.p, h_units.data(), h_units.size() * sizeof(Unit));
     up(d_ubase.p, ubase.data(), ubase.size() * sizeof(int));
-    up(d_gid.p, gid.data(), nn * sizeof(int));
-    if (NI > 0) up(d_gi.p, gi.data(), (size_t) NI * sizeof(int));
-    up(d_pos.p, pos.data(), nn * sizeof(V2));
-    if (Acc<R>::separate) up(d_acc.p, acc.data(), nn * sizeof(double2));
-    up(d_size.p, size.data(), nn * sizeof(R));
-    up(d_fixed.p, fixed.data(), nn);
+    up(d_gid.p, gid.get(), nn * sizeof(int));
+    if (NI > 0) up(d_gi.p, gi.get(), (size_t) NI * sizeof(int));
+    up(d_pos.p, pos.get(), nn * sizeof(V2));
+    if (Acc<R>::separate) up(d_acc.p, acc.get(), nn * sizeof(double2));
+    up(d_size.p, size.get(), nn * sizeof(R));
+    up(d_fixed.p, fixed.get(), nn);
     if (from_edges) {
         if (hs.ok()) {
             k_pack_node4<R><<<ceil_div(N, 256), 256, 0, stream>>>(N, d_pos.p, d_mass.p, d_size.p,
@@ -2199,13 +2319,11 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
             hs.check(hipGetLastError(), "launch");
         }
     } else {
-        std::vector<V4> node4(nn);
-        for (size_t i = 0; i < nn; i++) node4[i] = mk4<R>(pos[i].x, pos[i].y, mass[i], size[i]);
-        up(d_node4.p, node4.data(), nn * sizeof(V4));
-        up(d_mass.p, mass.data(), nn * sizeof(R));
+        up(d_node4.p, node4.get(), nn * sizeof(V4));
+        up(d_mass.p, mass.get(), nn * sizeof(R));
         up(d_inc_off.p, inc_off.data(), (nn + 1) * sizeof(int));
-        up(d_inc_node.p, inc_node.data(), (size_t) I * sizeof(int));
-        up(d_coef.p, coef.data(), (size_t) I * sizeof(R));
+        up(d_inc_node.p, inc_node.get(), (size_t) I * sizeof(int));
+        up(d_coef.p, coef.get(), (size_t) I * sizeof(R));
     }
     for (int c = 0; c < 2; c++) up(d_class[c].p, h_class[c].data(), h_class[c].size() * sizeof(int4));
     up(d_items.p, h_items.data(), h_items.size() * sizeof(int4));
@@ -2214,11 +2332,26 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     if (hs.ok())
         hs.check(hipMemsetAsync(d_psort.p + nn, 0, kLeafSize * sizeof(V4), stream), "memset");
     if (hs.ok()) hs.check(hipMemsetAsync(d_F[0].p, 0, nn * sizeof(double2), stream), "memset");
-    if (hs.ok()) hs.check(hipMemsetAsync(d_F[1].p, 0, nn * sizeof(double2), stream), "memset");
+    // the first iteration reads the previous force from F[1]: zero, or the
+    // force a resumed graph left
+    std::vector<double2> fold;
+    if (any_fold) {
+        fold.assign(nn, make_double2(0.0, 0.0));
+        host_parallel(G, host_threads_for(N), [&](int g) {
+            const double *f = gr[g].fold;
+            const int n = h_gd[g].n, o = h_gd[g].noff;
+            for (int i = 0; f && i < n; i++) fold[(size_t) o + i] = make_double2(f[i], f[n + i]);
+        });
+        up(d_F[1].p, fold.data(), nn * sizeof(double2));
+    } else if (hs.ok()) {
+        hs.check(hipMemsetAsync(d_F[1].p, 0, nn * sizeof(double2), stream), "memset");
+    }
     if (hs.ok()) hs.check(hipMemsetAsync(d_Fa.p, 0, nn * sizeof(double2), stream), "memset");
     // the attraction's stream (default priority) and the events that fork it
-    // from the batch's stream and join it back
-    if (hs.ok() && !s_att) hs.check(hipStreamCreateWithFlags(&s_att, hipStreamNonBlocking), "stream");
+    // from the batch's stream and join it back (not made without overlap:
+    // an idle stream still takes a share of the hardware queues)
+    if (hs.ok() && overlap && !s_att)
+        hs.check(hipStreamCreateWithFlags(&s_att, hipStreamNonBlocking), "stream");
     if (hs.ok() && !ev_fork) hs.check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "event");
     if (hs.ok() && !ev_join) hs.check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "event");
     if (hs.ok()) {
@@ -2978,6 +3111,73 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
     if (chosen) *chosen = pick;
     if (times)
         for (int k = 0; k < FA2_TUNE_NTIMES; k++) times[k] = t[k];
+    return 0;
+}
+
+template <typename R>
+int EngineImpl<R>::read_states(int count, const int *graphs, double *const *pos,
+                               double *const *fold, double (*sp)[4])
+{
+    int lo = N, hi = 0; // the node span of the graphs asked for
+    bool want_pos = false, want_fold = false;
+    for (int k = 0; k < count; k++) {
+        const int g = graphs[k];
+        if (g < 0 || g >= G) {
+            set_message("no graph %d in this batch", g);
+            return 1;
+        }
+        lo = std::min(lo, h_gd[g].noff);
+        hi = std::max(hi, h_gd[g].noff + h_gd[g].n);
+        want_pos = want_pos || (pos && pos[k]);
+        want_fold = want_fold || (fold && fold[k]);
+    }
+    if (count <= 0) return 0;
+    (void) hipSetDevice(device);
+    const size_t span = (size_t) (hi - lo);
+    std::vector<double2> acc_h(want_pos && Acc<R>::separate ? span : 0);
+    std::vector<V2> pos_h(want_pos && !Acc<R>::separate ? span : 0);
+    std::vector<double2> f_h(want_fold ? span : 0);
+    std::vector<GraphState> gs_h(sp ? G : 0);
+    hipError_t e = hipSuccess;
+    auto down = [&](void *dst, const void *src, size_t bytes) {
+        if (e == hipSuccess && bytes)
+            e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream);
+    };
+    down(acc_h.data(), d_acc.p + lo, acc_h.size() * sizeof(double2));
+    down(pos_h.data(), d_pos.p + lo, pos_h.size() * sizeof(V2));
+    // the last force is in F[1 - parity]
+    down(f_h.data(), d_F[1 - parity].p + lo, f_h.size() * sizeof(double2));
+    down(gs_h.data(), d_gs.p, gs_h.size() * sizeof(GraphState));
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        set_hip_error("could not fetch the state of the batch from the device", e);
+        return 1;
+    }
+    for (int k = 0; k < count; k++) {
+        const GraphDesc &D = h_gd[graphs[k]];
+        const size_t n = (size_t) D.n, o = (size_t) (D.noff - lo);
+        if (pos && pos[k]) {
+            double *out = pos[k];
+            for (size_t i = 0; i < n; i++) {
+                out[i] = Acc<R>::separate ? acc_h[o + i].x : (double) pos_h[o + i].x;
+                out[n + i] = Acc<R>::separate ? acc_h[o + i].y : (double) pos_h[o + i].y;
+            }
+        }
+        if (fold && fold[k]) {
+            double *out = fold[k];
+            for (size_t i = 0; i < n; i++) {
+                out[i] = f_h[o + i].x;
+                out[n + i] = f_h[o + i].y;
+            }
+        }
+        if (sp) {
+            const GraphState &st = gs_h[graphs[k]];
+            sp[k][0] = st.tsw;
+            sp[k][1] = st.ttr;
+            sp[k][2] = st.speed;
+            sp[k][3] = st.eff;
+        }
+    }
     return 0;
 }
 

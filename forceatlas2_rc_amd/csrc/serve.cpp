@@ -130,6 +130,11 @@ std::string fa2_server::open_engine(Running *r, const std::vector<Job *> &jobs, 
         ps[k].iterations = jobs[k]->p.iterations - jobs[k]->done_before;
         r->longest = std::max(r->longest, ps[k].iterations);
         gr[k] = EngineGraph{&ps[k], jobs[k]->pos.data(), has_edges ? &jobs[k]->edges : nullptr};
+        if (jobs[k]->done_before > 0) { // resumes from its saved state
+            gr[k].fold = jobs[k]->fold.data();
+            gr[k].speed = jobs[k]->speed;
+            gr[k].eff = jobs[k]->eff;
+        }
     }
     r->e = make_engine(use_double);
     if (!r->e) return "out of memory";
@@ -142,12 +147,6 @@ std::string fa2_server::open_engine(Running *r, const std::vector<Job *> &jobs, 
     // would hold the lane between chunks)
     r->e->overlap = 0;
     if (r->e->open(gr.data(), (int) G) != 0) return fa2_gpu_error();
-    for (size_t k = 0; k < G; k++) {
-        const Job *j = jobs[k];
-        if (j->done_before > 0 &&
-            r->e->set_state((int) k, j->pos.data(), j->fold.data(), j->speed, j->eff) != 0)
-            return fa2_gpu_error();
-    }
     r->jobs = jobs;
     r->finished.assign(G, 0);
     r->target = 0;
@@ -209,22 +208,23 @@ bool fa2_server::refill(Running *r, const std::vector<int> &done)
     }
     r->bytes = keep_bytes + take_bytes;
     const double t_start = now_s();
-    // the remaining requests' state
+    // the remaining requests' state, in one transfer per array
     std::string err;
-    for (size_t q = 0; q < keep.size() && err.empty(); q++) {
+    const size_t K = keep.size();
+    std::vector<double *> kpos(K), kfold(K);
+    std::vector<double[4]> ksp(K);
+    for (size_t q = 0; q < K; q++) {
+        keep[q]->fold.resize(2 * (size_t) keep[q]->p.n);
+        kpos[q] = keep[q]->pos.data();
+        kfold[q] = keep[q]->fold.data();
+    }
+    if (r->e->read_states((int) K, keep_k.data(), kpos.data(), kfold.data(), ksp.data()) != 0)
+        err = fa2_gpu_error();
+    for (size_t q = 0; q < K && err.empty(); q++) {
         Job *j = keep[q];
-        const int k = keep_k[q];
-        const size_t n = (size_t) j->p.n;
-        double sp[4];
-        j->fold.resize(2 * n);
-        if (r->e->positions(k, j->pos.data()) != 0 || r->e->forces(k, j->fold.data()) != 0 ||
-            r->e->speed_state(k, sp) != 0) {
-            err = fa2_gpu_error();
-            break;
-        }
-        j->speed = sp[2];
-        j->eff = sp[3];
-        j->done_before += done[k];
+        j->speed = ksp[q][2];
+        j->eff = ksp[q][3];
+        j->done_before += done[keep_k[q]];
     }
     delete r->e; // waits for its stream, frees in the stream's order
     r->e = nullptr;
@@ -343,6 +343,27 @@ bool fa2_server::advance(Running *r)
     std::vector<int> done(G), status(G);
     std::string err;
     if (r->e->progress(done.data(), status.data()) != 0) err = fa2_gpu_error();
+    // the positions of the graphs that reached their iteration count, in one
+    // transfer
+    std::vector<int> fin;
+    std::vector<std::vector<double>> fpos;
+    std::vector<double *> fptr;
+    int frc = 0;
+    if (err.empty()) {
+        for (int k = 0; k < G; k++)
+            if (!r->finished[k] && status[k] == 0 &&
+                r->jobs[k]->done_before + done[k] >= r->jobs[k]->p.iterations)
+                fin.push_back(k);
+        fpos.resize(fin.size());
+        for (size_t q = 0; q < fin.size(); q++) {
+            fpos[q].resize(2 * (size_t) r->jobs[fin[q]]->p.n);
+            fptr.push_back(fpos[q].data());
+        }
+        if (!fin.empty())
+            frc = r->e->read_states((int) fin.size(), fin.data(), fptr.data(), nullptr, nullptr);
+    }
+    std::string ferr = frc ? fa2_gpu_error() : std::string();
+    size_t fq = 0;
     bool all = true;
     for (int k = 0; k < G; k++) {
         if (r->finished[k]) continue;
@@ -368,16 +389,15 @@ bool fa2_server::advance(Running *r)
             all = false;
             continue;
         }
-        std::vector<double> out(2 * (size_t) j->p.n);
-        const int rc = r->e->positions(k, out.data());
-        if (rc == 0) (void) r->e->retire(k);
+        std::vector<double> &out = fpos[fq++];
+        if (frc == 0) (void) r->e->retire(k);
         std::lock_guard<std::mutex> lk(mu);
-        if (rc == 0) {
+        if (frc == 0) {
             j->pos.swap(out);
             j->state = kDone;
         } else {
             j->state = kFailed;
-            j->error = fa2_gpu_error();
+            j->error = ferr;
         }
         j->t_done = now_s();
         r->finished[k] = 1;
