@@ -87,6 +87,28 @@ def calibration_copy(nbytes):
     log(f"calibration copy of {nbytes} bytes done")
 
 
+def sort_yardstick(keys, engine_ms, reps=50):
+    """rocPRIM's radix_sort_pairs (libfa2yard.so, built beside the library
+    and never part of it) on the same 2^21 (Morton code, node) pairs as the
+    engine's sort, all 32 bits: a stated yardstick for the engine's sort
+    phase, which also keeps each graph of a batch apart and gathers the
+    sorted node records in its last pass."""
+    import ctypes
+    path = os.path.join(ROOT, "forceatlas2_rc_amd", "libfa2yard.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    f = lib.fa2yard_radix_sort_pairs
+    f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+    keys = np.ascontiguousarray(keys, np.uint32)
+    ms, ok = ctypes.c_double(), ctypes.c_int()
+    if f(0, int(keys.size), keys.ctypes.data, reps, ctypes.byref(ms), ctypes.byref(ok)):
+        return None
+    return dict(rocprim_radix_sort_pairs_ms=ms.value, stable_order=bool(ok.value),
+                engine_sort_phase_ms=engine_ms, pairs=int(keys.size), reps=reps)
+
+
 def host_threads():
     """Threads for the CPU baseline: every core this process may run on,
     within the OpenMP thread count the host grants it (OMP_NUM_THREADS, when
@@ -469,6 +491,8 @@ def main():
     ap.add_argument("--no-batch-alone", action="store_true",
                     help="skip laying out each graph of the batch alone")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-yardstick", action="store_true",
+                    help="skip timing rocPRIM's radix sort on the C3 codes")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-sub-requests", type=int, default=40,
                     help="requests of the stream replayed below capacity")
@@ -567,6 +591,13 @@ def main():
     work = b.count_work()
     pos_out = b.positions(0)
     finite = bool(np.all(np.isfinite(pos_out)))
+    yard = None
+    if not args.no_yardstick:
+        tr = b.tree(0)
+        keys_in = np.empty(n, np.uint32)  # the codes in node order, as the sort takes them
+        keys_in[tr["order"].astype(np.int64)] = tr["keys"]
+        yard = sort_yardstick(keys_in, stages["sort"] * 1e3)
+        log("sort yardstick:", yard)
     b.close()
 
     A, O_, P, steps = work
@@ -612,6 +643,7 @@ def main():
                            replay_equals_eager=bool(np.array_equal(pos_replay, pos_out)),
                            layout_progression=progression,
                            positions_finite=finite,
+                           sort_yardstick=yard,
                            ingest=dict(host_numpy_prepare_s=t_prep, upload_prepared_s=t_up,
                                        device_prepare_with_upload_s=t_dev)),
                roofline=roof, iteration_roofline=iter_roof)
