@@ -1444,16 +1444,16 @@ __device__ inline void attract_term(typename Vec<R>::v2 pi, typename Vec<R>::v2 
 // The incidences e0, e0 + step, ... < e1 of one node, in that order. Each
 // round first loads the list entries of U incidences, then their U
 // positions, then adds their terms; the index of a slot past e1 is clamped
-// to e1 - 1 (a valid entry), so every load is unconditional. With UNIFORM
-// every coefficient of the node equals c0 and none is loaded.
+// to `last` (a valid entry: the list's last, or in the sliced layout the
+// lane's own last), so every load is unconditional. With UNIFORM every
+// coefficient of the node equals c0 and none is loaded.
 template <typename R, int U, bool UNIFORM>
-__device__ inline void attract_terms(int e0, int e1, int step, bool linlog, bool adjust,
+__device__ inline void attract_terms(int e0, int e1, int step, int last, bool linlog, bool adjust,
                                      typename Vec<R>::v2 pi, R si,
                                      const typename Vec<R>::v2 *__restrict__ pos,
                                      const R *__restrict__ size, const int *__restrict__ inc_node,
                                      const R *__restrict__ inc_coef, R c0, ForceSum<R> &sum)
 {
-    const int last = e1 - 1;
     for (int e = e0; e < e1; e += U * step) {
         int j[U];
         R c[U];
@@ -1495,6 +1495,11 @@ template <int GL> __device__ inline void group_sum(double &x, double &y)
 constexpr int kAttractXcdRun = 32;
 
 // GL lanes per node; the work list holds (node, first incidence, end, graph).
+// The one-lane class (GL = 1) reads its lists in the sliced layout: the 64
+// lanes of a wave have their r-th entries side by side (entry r of lane l at
+// slice + 64 r + l), so that each load of list entries is one contiguous
+// 256-byte access; its work entries hold (node, slice + l, slice + l + 64 deg,
+// graph), and the lane's entries are those of its incidence list, in order.
 template <typename R, int GL, int U>
 __global__ void __launch_bounds__(256)
 k_attract(int count, const int4 *__restrict__ work, const GraphDesc *__restrict__ gd,
@@ -1509,17 +1514,39 @@ k_attract(int count, const int4 *__restrict__ work, const GraphDesc *__restrict_
     const int i = wk.x;
     const int flags = gd[wk.w].flags;
     const bool live = slot < count && !gs[wk.w].retired;
+    constexpr int step = GL == 1 ? 64 : GL;
+    const int last = GL == 1 ? wk.z - 64 : wk.z - 1;
     ForceSum<R> sum;
     if (flags & GF_UNIFORM)
-        attract_terms<R, U, true>(wk.y + sub, wk.z, GL, flags & GF_LINLOG, flags & GF_ADJUST,
-                                  pos[i], size[i], pos, size, inc_node, inc_coef, inc_coef[wk.y],
-                                  sum);
+        attract_terms<R, U, true>(wk.y + sub, wk.z, step, last, flags & GF_LINLOG,
+                                  flags & GF_ADJUST, pos[i], size[i], pos, size, inc_node,
+                                  inc_coef, inc_coef[wk.y], sum);
     else
-        attract_terms<R, U, false>(wk.y + sub, wk.z, GL, flags & GF_LINLOG, flags & GF_ADJUST,
-                                   pos[i], size[i], pos, size, inc_node, inc_coef, (R) 0, sum);
+        attract_terms<R, U, false>(wk.y + sub, wk.z, step, last, flags & GF_LINLOG,
+                                   flags & GF_ADJUST, pos[i], size[i], pos, size, inc_node,
+                                   inc_coef, (R) 0, sum);
     group_sum<GL>(sum.x, sum.y);
     if (!live || sub != 0) return;
     Fa[i] = make_double2(sum.x, sum.y);
+}
+
+// The one-lane class's lists copied into the sliced layout (see k_attract):
+// lane p's entries go to slice + 64 r + lane, r < deg; the rest of a slice
+// is never read (a lane clamps its loads to its own last entry).
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_sell_fill(int count, const int4 *__restrict__ work, const int *__restrict__ inc_off,
+            const int *__restrict__ inc_node, const R *__restrict__ inc_coef,
+            int *__restrict__ sell_node, R *__restrict__ sell_coef)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= count) return;
+    const int4 wk = work[p];
+    const int e0 = inc_off[wk.x], deg = (wk.z - wk.y) / 64;
+    for (int r = 0; r < deg; r++) {
+        sell_node[wk.y + 64 * r] = inc_node[e0 + r];
+        sell_coef[wk.y + 64 * r] = inc_coef[e0 + r];
+    }
 }
 
 // One wave per item (node, first incidence, end, partial slot or -1).
@@ -1541,12 +1568,13 @@ k_attract_items(int count, const int4 *__restrict__ items, const int *__restrict
     const int flags = gd[g].flags;
     ForceSum<R> sum;
     if (flags & GF_UNIFORM)
-        attract_terms<R, U, true>(it.y + lane, it.z, 64, flags & GF_LINLOG, flags & GF_ADJUST,
-                                  pos[i], size[i], pos, size, inc_node, inc_coef, inc_coef[it.y],
-                                  sum);
+        attract_terms<R, U, true>(it.y + lane, it.z, 64, it.z - 1, flags & GF_LINLOG,
+                                  flags & GF_ADJUST, pos[i], size[i], pos, size, inc_node,
+                                  inc_coef, inc_coef[it.y], sum);
     else
-        attract_terms<R, U, false>(it.y + lane, it.z, 64, flags & GF_LINLOG, flags & GF_ADJUST,
-                                   pos[i], size[i], pos, size, inc_node, inc_coef, (R) 0, sum);
+        attract_terms<R, U, false>(it.y + lane, it.z, 64, it.z - 1, flags & GF_LINLOG,
+                                   flags & GF_ADJUST, pos[i], size[i], pos, size, inc_node,
+                                   inc_coef, (R) 0, sum);
     group_sum<64>(sum.x, sum.y);
     if (lane != 0 || gs[g].retired) return;
     if (it.w < 0) {
@@ -1847,6 +1875,10 @@ template <typename R> struct EngineImpl final : Engine {
     DevBuf<uint8_t> d_fixed;
     DevBuf<int> d_inc_off, d_inc_node;
     DevBuf<int4> d_class[2];
+    // the one-lane class's lists in the sliced layout (k_attract), and the
+    // first entry of each wave's slice (one more at the end)
+    DevBuf<int> d_sell_node, d_sell_base;
+    DevBuf<R> d_sell_coef;
     DevBuf<int4> d_items, d_hubs;
     DevBuf<double2> d_hubpart;
     DevBuf<V4> d_tbox, d_gbox, d_psort, d_nd;
@@ -2238,6 +2270,32 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
             h_hubs[at_hubs[k] + q] = h;
         }
     });
+    // the one-lane class in the sliced layout: the slice of wave w (work
+    // entries 64 w .. 64 w + 63) holds 64 x (its longest list) entries; a
+    // work entry becomes (node, slice + lane, slice + lane + 64 deg, graph)
+    const size_t n0 = h_class[0].size(), nw0 = (n0 + 63) / 64;
+    std::vector<int> sell_base(nw0 + 1, 0);
+    {
+        long long at = 0;
+        for (size_t w = 0; w < nw0; w++) {
+            int longest = 0;
+            for (size_t p = 64 * w; p < std::min(n0, 64 * w + 64); p++)
+                longest = std::max(longest, h_class[0][p].z - h_class[0][p].y);
+            sell_base[w] = (int) at;
+            at += 64LL * longest;
+            if (at > INT_MAX - 64) {
+                set_message("too many incidences for one batch");
+                return 1;
+            }
+        }
+        sell_base[nw0] = (int) at;
+        for (size_t p = 0; p < n0; p++) {
+            int4 &wk = h_class[0][p];
+            const int deg = wk.z - wk.y;
+            wk.y = sell_base[p / 64] + (int) (p % 64);
+            wk.z = wk.y + 64 * deg;
+        }
+    }
 
     HipStatus hs;
     auto up = [&](void *dst, const void *src, size_t bytes) {
@@ -2263,6 +2321,9 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     hs.check(d_size.alloc(nn, ast), "alloc");
     hs.check(d_fixed.alloc(nn, ast), "alloc");
     for (int c = 0; c < 2; c++) hs.check(d_class[c].alloc(h_class[c].size(), ast), "alloc");
+    hs.check(d_sell_node.alloc(sell_base[nw0], ast), "alloc");
+    hs.check(d_sell_coef.alloc(sell_base[nw0], ast), "alloc");
+    hs.check(d_sell_base.alloc(nw0 + 1, ast), "alloc");
     hs.check(d_items.alloc(h_items.size(), ast), "alloc");
     hs.check(d_hubs.alloc(h_hubs.size(), ast), "alloc");
     hs.check(d_hubpart.alloc(nslots, ast), "alloc");
@@ -2326,6 +2387,13 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         up(d_coef.p, coef.get(), (size_t) I * sizeof(R));
     }
     for (int c = 0; c < 2; c++) up(d_class[c].p, h_class[c].data(), h_class[c].size() * sizeof(int4));
+    up(d_sell_base.p, sell_base.data(), (nw0 + 1) * sizeof(int));
+    if (hs.ok() && n0 > 0) { // after the incidence lists, uploaded or built above
+        k_sell_fill<R><<<ceil_div((int) n0, 256), 256, 0, stream>>>(
+            (int) n0, d_class[0].p, d_inc_off.p, d_inc_node.p, d_coef.p, d_sell_node.p,
+            d_sell_coef.p);
+        hs.check(hipGetLastError(), "launch");
+    }
     up(d_items.p, h_items.data(), h_items.size() * sizeof(int4));
     up(d_hubs.p, h_hubs.data(), h_hubs.size() * sizeof(int4));
     for (int c = 0; c < 3; c++) up(d_rowg[c].p, h_rowg[c].data(), h_rowg[c].size() * sizeof(int));
@@ -2391,7 +2459,8 @@ void EngineImpl<R>::launch_attract_class(int c, int u, double2 *Fa, hipStream_t 
     do {                                                                                        \
         if (c == 0)                                                                             \
             k_attract<R, 1, U><<<ceil_div(cnt, 256), 256, 0, st>>>(                         \
-                cnt, d_class[0].p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_node.p, d_coef.p, Fa); \
+                cnt, d_class[0].p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_sell_node.p,            \
+                d_sell_coef.p, Fa);                                                             \
         else if (c == 1)                                                                        \
             k_attract<R, 16, U><<<ceil_div(cnt, 16), 256, 0, st>>>(                         \
                 cnt, d_class[1].p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_node.p, d_coef.p, Fa); \
@@ -3253,6 +3322,10 @@ template <typename R> static size_t footprint_of(long long n, long long incidenc
     b += (size_t) units * (sizeof(Unit) + (kUnitCap + kCountStride) * sizeof(int));
     b += (size_t) n * sizeof(TNode<R>); // the leaves' traversal records
     b += (size_t) incidences * (sizeof(int) + sizeof(R));
+    // the one-lane class's sliced lists: a lane's slot count is its wave's
+    // longest list, at most 16, and at most 15 more than its own
+    const long long sliced = std::min(16 * n, incidences + 15 * n) + 1024;
+    b += (size_t) sliced * (sizeof(int) + sizeof(R)) + (size_t) (n / 64 + 2) * sizeof(int);
     b += (size_t) items * (2 * sizeof(int4) + sizeof(double2));
     b += (size_t) tiles * (sizeof(Tile) + 2 * sizeof(V4) + sizeof(double2) + 256 * sizeof(uint32_t));
     b += sizeof(GraphDesc) + sizeof(GraphState) + 256 * sizeof(uint32_t) +
