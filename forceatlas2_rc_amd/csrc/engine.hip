@@ -881,7 +881,6 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
             s_sum[4][q] = r.m;
             s_sum[5][q] = r.mx;
             s_sum[6][q] = r.my;
-            tn[t].c = region_record<R>(r);
             const int p = s_parent[q];
             if (p >= 0) {
                 const int pq = p - tbase;
@@ -894,8 +893,18 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
             t = p;
         }
     }
-    // the skip links of the regions merged here
-    if (inside) tn[t].l.y = l.w == gend - 1 ? kNoRecord : record_of(esc_last, NI);
+    // the traversal records of the regions merged here, whole, one per
+    // thread in region order (coalesced 32-byte writes): the summary from
+    // LDS, the links of k_karras and the skip link
+    __syncthreads();
+    if (inside) {
+        const Summ<R> r{s_sum[0][q], s_sum[1][q], s_sum[2][q], s_sum[3][q],
+                        s_sum[4][q], s_sum[5][q], s_sum[6][q]};
+        TNode<R> rec;
+        rec.c = region_record<R>(r);
+        rec.l = make_int4(l.x, l.w == gend - 1 ? kNoRecord : record_of(esc_last, NI), l.z, l.w);
+        tn[t] = rec;
+    }
 }
 
 // One unit of level `level` >= 2: its listed regions are staged in LDS, one
@@ -1004,8 +1013,12 @@ k_aggregate_unit(const Unit *__restrict__ units, int level, const GraphDesc *__r
     if (q < cnt) {
         const int t = s_t[q];
         const Summ<R> r = get(q);
-        tn[t].c = region_record<R>(r);
-        tn[t].l.y = s_skip[q];
+        // the whole record in one 32-byte write
+        TNode<R> rec;
+        rec.l = tn[t].l;
+        rec.l.y = s_skip[q];
+        rec.c = region_record<R>(r);
+        tn[t] = rec;
         if (s_ps[q] == -1) { // merged by a higher unit
             nd[2 * t] = mk4<R>(r.x0, r.y0, r.x1, r.y1);
             nd[2 * t + 1] = mk4<R>(r.m, r.mx, r.my, (R) 0);
