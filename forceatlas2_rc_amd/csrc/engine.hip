@@ -1308,7 +1308,7 @@ __device__ __forceinline__ void repulse_out(const GraphDesc &G, typename Vec<R>:
             fy -= (double) (me.y * gg);
         }
     }
-    F[dst] = make_double2(fx, fy);
+    st_stream(F + dst, fx, fy);
 }
 
 // The traversal with two walks per wave: lanes hold node k of the first 64
@@ -1473,8 +1473,8 @@ __device__ inline void attract_terms(int e0, int e1, int step, int last, bool li
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int q = min(e + u * step, last);
-            j[u] = inc_node[q];
-            c[u] = UNIFORM ? c0 : inc_coef[q];
+            j[u] = ld_stream(inc_node + q);
+            c[u] = UNIFORM ? c0 : ld_stream(inc_coef + q);
         }
         typename Vec<R>::v2 pj[U];
         R sj[U];

@@ -35,6 +35,19 @@ template <typename R> __host__ __device__ inline typename Vec<R>::v4 mk4(R a, R 
     return v;
 }
 
+// Loads and stores of data that streams through a kernel once (the
+// attraction's incidence lists, the forces the traversal leaves for the next
+// phase), with the non-temporal hint: the lines do not displace what the
+// kernels running beside them reuse from L2 (the traversal's records near the
+// tree's top, the gathered positions). Only the cache policy differs.
+template <typename T> __device__ inline T ld_stream(const T *p) { return __builtin_nontemporal_load(p); }
+__device__ inline void st_stream(double2 *p, double x, double y)
+{
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const d2v v = {x, y};
+    __builtin_nontemporal_store(v, (d2v *) p);
+}
+
 // Reciprocal for the bulk force arithmetic of a positive, finite x:
 // v_rcp_f32 (1 ulp) in FP32; in FP64, v_rcp_f64 refined by two
 // Newton-Raphson steps in fused multiply-adds (each step squares the
