@@ -1654,7 +1654,7 @@ k_swing(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
     for (int i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
         double2 f = F[i];
         if (live) {
-            const double2 fa = Fa[i];
+            const double2 fa = ld_stream(Fa + i);
             f.x += fa.x;
             f.y += fa.y;
             F[i] = f;
@@ -1769,9 +1769,9 @@ k_apply(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
     for (int i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
         typename Vec<R>::v2 p = pos[i];
         if (move && !(use_fixed && fixed[i])) {
-            const double2 f = F[i];
+            const double2 f = ld_stream(F + i);
             double sw, tr;
-            swing_node(f, Fold[i], (double) mass[i], sw, tr);
+            swing_node(f, ld_stream(Fold + i), (double) mass[i], sw, tr);
             double factor;
             if (adjust) {
                 const double fx = (double) f.x, fy = (double) f.y;
@@ -1786,10 +1786,10 @@ k_apply(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
                 factor = speed / (1.0 + __builtin_sqrt(speed * sw));
             }
             if (Acc<R>::separate) {
-                double2 q = acc[i];
+                double2 q = ld_stream(acc + i);
                 q.x += (double) f.x * factor;
                 q.y += (double) f.y * factor;
-                acc[i] = q;
+                st_stream(acc + i, q.x, q.y);
                 p = mk2<R>((R) q.x, (R) q.y);
             } else {
                 p.x += (R) ((double) f.x * factor);

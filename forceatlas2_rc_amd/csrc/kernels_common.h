@@ -37,10 +37,17 @@ template <typename R> __host__ __device__ inline typename Vec<R>::v4 mk4(R a, R 
 
 // Loads and stores of data that streams through a kernel once (the
 // attraction's incidence lists, the forces the traversal leaves for the next
-// phase), with the non-temporal hint: the lines do not displace what the
+// phase, the forces and accumulators of the displacement), with the
+// non-temporal hint: the lines do not displace what the
 // kernels running beside them reuse from L2 (the traversal's records near the
 // tree's top, the gathered positions). Only the cache policy differs.
 template <typename T> __device__ inline T ld_stream(const T *p) { return __builtin_nontemporal_load(p); }
+__device__ inline double2 ld_stream(const double2 *p)
+{
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const d2v v = __builtin_nontemporal_load((const d2v *) p);
+    return make_double2(v.x, v.y);
+}
 __device__ inline void st_stream(double2 *p, double x, double y)
 {
     typedef double d2v __attribute__((ext_vector_type(2)));
