@@ -6,7 +6,9 @@
   (pipeline.cl:105-270); one iteration's forces within 1e-5 normwise of the
   FP64 oracle, over the graph and within each attraction class; iterations
   replayed from the captured HIP graph (with the autotuner's shapes) equal to
-  kernel-by-kernel launches bit for bit; finite positions.
+  kernel-by-kernel launches bit for bit; finite positions. In double
+  precision the tree is bit-exact and the forces within 1e-12 of the FP64
+  oracle.
 * 512 stochastic-block-model graphs in one batch: each graph's layout alone
   equals its layout in the batch bit for bit, and the smallest, median and
   largest graphs' first-iteration forces in the batch are within 1e-5 of the
@@ -67,6 +69,30 @@ def test_rmat21_tree_and_forces(rmat21):
     iso = deg == 0  # half the nodes have no edges: repulsion and gravity only
     assert iso.sum() > n // 4 and normwise(f[iso], fr[iso]) <= 1e-5
     assert sp == pytest.approx(tr["speed"][0], rel=1e-5)
+
+
+def test_rmat21_double_tree_and_forces(rmat21):
+    """The same workload in double precision: the Morton codes of the FP64
+    positions, the permutation and the topology bit-exact, and one
+    iteration's forces within 1e-12 normwise of the FP64 oracle (mode d),
+    over the graph and within each attraction class."""
+    n, e, pos, g, deg = rmat21
+    with Batch([g], precision="double", autotune=False) as b:
+        b.run(1)
+        b.sync()
+        tree = b.tree(0)
+        f = b.forces(0)
+    ref = O.large_tree(pos[:, 0], pos[:, 1], "d")
+    for k in ("keys", "order", "child", "range", "parent"):
+        assert np.array_equal(tree[k].astype(np.int64), ref[k].astype(np.int64)), k
+    del tree, ref
+    _, tr = O.large_run(n, e, pos, 1, mode="d", theta=1.2, linlog=True, outbound=True,
+                        trace=True)
+    fr = tr["force"][0]
+    assert normwise(f, fr) <= 1e-12
+    for lo, hi in CLASSES:
+        sel = (deg >= lo) & (deg <= hi)
+        assert normwise(f[sel], fr[sel]) <= 1e-12, (lo, hi)
 
 
 def test_rmat21_replay_equals_eager(rmat21):

@@ -150,3 +150,22 @@ def test_waiting_requests_join_a_running_batch():
     for g, pos in zip([long_] + late, res):
         assert np.array_equal(pos, _alone(g))
     assert np.array_equal(res_other, _alone(other, "double"))
+
+
+def test_requests_from_edge_lists_join_a_running_batch():
+    """Continuous batching of requests whose incidence lists the device
+    builds from their edge lists (fa2_serve_submit_edges): rebuilt batches
+    prepare their newcomers on the device beside the resumed requests, and
+    every result equals the layout alone bit for bit."""
+    long_ = _graph(2500, 61, iterations=300, linlog=True, dissuade_hubs=True)
+    short = _graph(700, 62, iterations=10)
+    late = [_graph(1800, 63, iterations=40), _graph(900, 64, iterations=70, adjust=True, size=1.2)]
+    with Server(max_batches=1, chunk=5) as srv:
+        i_long, i_short = srv.submit(long_, ingest="device"), srv.submit(short, ingest="device")
+        srv.result(i_short)
+        ids = [srv.submit(g, ingest="device") for g in late]
+        res = [srv.result(j)[0] for j in [i_long] + ids]
+        st = srv.stats()
+    assert st["refills"] >= 1
+    for g, pos in zip([long_] + late, res):
+        assert np.array_equal(pos, _alone(g))
