@@ -56,6 +56,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# The serving leg runs 8 streams at once. HIP gives a process 4 hardware
+# queues by default, so pairs of streams would share one, and a stream's
+# copies (a batch being rebuilt) would wait behind the kernels the other
+# stream queued. Read by the HIP runtime when it starts, before any GPU call;
+# a value set by the caller is kept.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 METRIC = ("FA2 node-iters/s at 2M nodes; served layouts/s + p99 vs "
           "one-at-a-time plain")
 HBM_PEAK_GBS = 8000.0
@@ -456,6 +463,7 @@ def serving_leg(args):
                 size_class_ceiling=dict(speedup=ceiling, classes=classes),
                 batches=stats["batches"], refills=stats["refills"],
                 refill_s=stats["refill_s"], uncaptured_batches=stats["uncaptured"],
+                hw_queues=os.environ.get("GPU_MAX_HW_QUEUES"),
                 max_batches_running=stats["peak_running"],
                 below_capacity=dict(rate=args.serve_sub_rate, requests=n_sub, span_s=span_sub,
                                     layouts_per_s=n_sub / span_sub,

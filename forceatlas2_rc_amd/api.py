@@ -571,6 +571,11 @@ class Server:
     and returns the positions (n, 2) and the times (queued, running, total)
     in seconds. Requests of both precisions can be mixed: each batch holds
     one precision.
+
+    Each of the ``max_batches`` streams should have a hardware queue of its
+    own: start the process with ``GPU_MAX_HW_QUEUES`` at least
+    ``max_batches`` (HIP's default is 4; the variable is read when the HIP
+    runtime starts). Streams that share a queue wait for each other's work.
     """
 
     def __init__(self, device=0, budget_bytes=0.0, max_batches=8, chunk=25):

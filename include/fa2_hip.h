@@ -239,8 +239,10 @@ typedef struct fa2_server fa2_server;
  * at once on their own streams, each advanced `chunk` iterations at a
  * time. When every stream is taken, waiting requests join a running batch
  * at a chunk boundary after some of its requests finished (the batch is
- * rebuilt with its remaining requests' state and the newcomers). NULL on
- * error. */
+ * rebuilt with its remaining requests' state and the newcomers). Each
+ * stream has a host thread of its own. The streams should have a hardware
+ * queue each: run the process with GPU_MAX_HW_QUEUES >= max_batches (the
+ * HIP runtime's default is 4). NULL on error. */
 fa2_server *fa2_serve_open(int device, double budget_bytes, int max_batches, int chunk);
 
 /* Queue one layout (arguments as for fa2_full_run, use_double choosing its
