@@ -59,9 +59,15 @@ sys.path.insert(0, ROOT)
 # The serving leg runs 8 streams at once. HIP gives a process 4 hardware
 # queues by default, so pairs of streams would share one, and a stream's
 # copies (a batch being rebuilt) would wait behind the kernels the other
-# stream queued. Read by the HIP runtime when it starts, before any GPU call;
-# a value set by the caller is kept.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# stream queued. Read by the HIP runtime when it starts, before any GPU call.
+# A caller's value of at least 16 is kept; a smaller one (environments often
+# export HIP's default of 4) is raised to 16.
+try:
+    _hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
+except ValueError:
+    _hwq = 0
+if _hwq < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 METRIC = ("FA2 node-iters/s at 2M nodes; served layouts/s + p99 vs "
           "one-at-a-time plain")

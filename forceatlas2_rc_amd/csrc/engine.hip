@@ -1059,14 +1059,12 @@ __device__ inline int xcd_block(int b, int nb, int run)
 // and accumulates the same terms in the same order.
 // ---------------------------------------------------------------------------
 
-// Per-lane force of the traversal. A region or node o contributes
-// scaling * m_i * M_o / d^2 * (xd, yd); the common factor scaling * m_i is
-// applied once at the end. The weight w = M_o / d^2 and the offsets are
-// computed in the working precision; each term xd * w is formed exactly in
-// FP64 (a product of two FP32 values fits in 53 bits) by a fused multiply-add
-// into the FP64 sum. A lane's sum therefore depends on its own terms only,
-// not on the other lanes of its wave: a graph gives the same result alone
-// and inside any batch.
+// Per-lane sum of the attraction's terms c_e * (x_j - x_i) (times the
+// linlog factor): the weight w and the offsets are computed in the working
+// precision, and each term xd * w is formed exactly in FP64 (a product of two
+// FP32 values fits in 53 bits) by a fused multiply-add into the FP64 sum. A
+// lane's sum depends on its own terms only, not on the other lanes of its
+// wave: a graph gives the same result alone and inside any batch.
 template <typename R> struct ForceSum {
     double x = 0.0, y = 0.0;
     __device__ inline void add(R xd, R yd, R w)
@@ -1081,25 +1079,79 @@ template <typename R> struct ForceSum {
 // With node sizes (prevent_overlap) a node pair's term is computed in FP64
 // from the FP32 inputs: the force jumps by orders of magnitude as two discs
 // touch, and the terms of overlapping discs are large and cancel. The term
-// goes straight into the FP64 total, without the factor scaling * m_i.
+// goes straight into an FP64 total, without the factor scaling * m_i.
 template <typename R>
 __device__ inline void lane_pair_adjust(typename Vec<R>::v4 me, typename Vec<R>::v4 o,
-                                        ForceSum<R> &sum, bool on)
+                                        double &sx, double &sy, bool on)
 {
     const double dx = (double) (me.x - o.x), dy = (double) (me.y - o.y);
     const double d = __builtin_sqrt(dx * dx + dy * dy) - ((double) me.w + (double) o.w);
     const double mo = (double) o.z;
     double f = d > 0.0 ? mo * frcp_r(d * d) : (d < 0.0 ? 100.0 * mo : 0.0);
     f = on ? f : 0.0;
-    sum.x += dx * f;
-    sum.y += dy * f;
+    sx += dx * f;
+    sy += dy * f;
+}
+
+// Per-lane force of the Barnes-Hut traversal, without the common factor
+// scaling * m_i. In single precision a graph adds each term xd * w by an FP32
+// fused multiply-add: the bulk force arithmetic is FP32, as the precision
+// model asks, and FP64 is kept for the global sums, the speed and the
+// position accumulators. A lane sums a few hundred terms at theta 1.2: over
+// the 2^21-node R-MAT workload the forces stay within 8e-7 of the FP64
+// oracle normwise in every degree class, against a tolerance of 1e-5. Two
+// kinds of graph keep an FP64 total, each term formed exactly in FP64 (a
+// product of two FP32 values fits in 53 bits): a graph with node sizes,
+// whose overlapping pairs give large terms that cancel, and a graph at
+// theta = 0, whose lanes sum every other node (FP32 sums of 2^14 terms
+// measured 5e-6). The kernels carry the FP64 totals only when some graph of
+// the batch needs them (FP64); a graph that does not keeps +0 there, so it
+// gets the same bits from either kernel. A lane's sum depends on its own
+// terms, in the traversal's order, and not on the other lanes of its wave,
+// so a graph gives the same result alone and inside any batch. In double
+// precision every total is FP64.
+template <typename R, bool FP64> struct LaneForce {
+    static constexpr bool kWide = FP64 && sizeof(R) == 4;
+    R x = (R) 0, y = (R) 0;
+    double wx = 0.0, wy = 0.0; // the FP64 total of a graph that needs one
+    __device__ inline void term(R xd, R yd, R w, bool wide)
+    {
+        if (kWide && wide) {
+            const double wd = (double) w; // exact products of FP32 values
+            wx = __builtin_fma((double) xd, wd, wx);
+            wy = __builtin_fma((double) yd, wd, wy);
+        } else {
+            x = fma_r(xd, w, x);
+            y = fma_r(yd, w, y);
+        }
+    }
+    // a node pair of a graph with sizes (which always has an FP64 total)
+    __device__ inline void pair_sized(typename Vec<R>::v4 me, typename Vec<R>::v4 o, bool on)
+    {
+        if (kWide) {
+            lane_pair_adjust<R>(me, o, wx, wy, on);
+        } else {
+            double sx = (double) x, sy = (double) y; // R is double here
+            lane_pair_adjust<R>(me, o, sx, sy, on);
+            x = (R) sx;
+            y = (R) sy;
+        }
+    }
+    __device__ inline double total_x() const { return kWide ? (double) x + wx : (double) x; }
+    __device__ inline double total_y() const { return kWide ? (double) y + wy : (double) y; }
+};
+
+// Whether graph G's traversal sums take the FP64 total (see LaneForce).
+__device__ inline bool fp64_sums(const GraphDesc &G)
+{
+    return (G.flags & GF_ADJUST) || G.theta2 == 0.0;
 }
 
 // With COUNT, the kernel also counts its work for the roofline: regions
 // accepted (A) and opened (O), and node pairs (P), into cnt[0..2].
 // psort holds kLeafSize records of padding after the last node, so that the
 // nodes of a small region are loaded at once.
-template <typename R, bool ANY_ADJUST, bool COUNT = false, bool WIDE = false>
+template <typename R, bool FP64_SUMS, bool COUNT = false, bool WIDE = false>
 __global__ void __launch_bounds__(1024)
 k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
           const GraphState *__restrict__ gs, const typename Vec<R>::v4 *__restrict__ psort,
@@ -1113,7 +1165,7 @@ k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restric
     const bool valid = k0 < N && !gs[gid[k]].retired;
     const int my_g = valid ? gid[k] : -1;
     const V4 me = psort[k];
-    ForceSum<R> sum;
+    LaneForce<R, FP64_SUMS> sum;
     // the lanes of one wave may belong to several graphs of a batch: walk
     // each graph's tree in turn with its own lanes
     uint64_t todo = __ballot(valid);
@@ -1123,7 +1175,8 @@ k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restric
         const bool mine = my_g == g;
         todo &= ~__ballot(mine);
         const GraphDesc G = gd[g];
-        const bool adjust = ANY_ADJUST && (G.flags & GF_ADJUST);
+        const bool adjust = FP64_SUMS && (G.flags & GF_ADJUST);
+        const bool wide = FP64_SUMS && fp64_sums(G);
         const R theta2 = (R) G.theta2;
         int done = mine ? INT_MIN + 1 : INT_MAX; // lanes of other graphs never take part
         int c = G.ioff;                          // the root (every graph has n >= 2)
@@ -1161,7 +1214,7 @@ k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restric
                 // the monopole, for the lanes that accept: a branch on the
                 // lane mask (the lanes' sums are updated in place)
                 asm volatile("");
-                sum.add(xd, yd, rc.z * frcp_r(d2));
+                sum.term(xd, yd, rc.z * frcp_r(d2), wide);
                 done = rl.w;
                 if (COUNT) nA++;
             }
@@ -1196,15 +1249,15 @@ k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restric
                     if (j0 + u > j1) break;
                     if (COUNT) nP += (j0 + u) != k;
                     if (adjust) {
-                        lane_pair_adjust<R>(me, o[u], sum, true);
+                        sum.pair_sized(me, o[u], true);
                     } else {
                         const R pxd = me.x - o[u].x, pyd = me.y - o[u].y;
                         const R pd2 = pxd * pxd + pyd * pyd;
                         R w = pd2 > (R) 0 ? o[u].z * frcp_r(pd2) : (R) 0;
-                        // select before the conversion to FP64: one select
-                        // instead of one per half of the double
+                        // the weight selected once, in a VGPR, before its
+                        // two fused multiply-adds
                         if (sizeof(R) == 4) asm volatile("" : "+v"(w));
-                        sum.add(pxd, pyd, w);
+                        sum.term(pxd, pyd, w, wide);
                     }
                 }
                 done = j1;
@@ -1217,7 +1270,7 @@ k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restric
         // repulsion minus gravity, the attraction kernels add the edges
         const GraphDesc G = gd[my_g];
         const double sscale = G.scaling * (double) me.z; // scaling * m_i in FP64
-        double fx = sscale * sum.x, fy = sscale * sum.y;
+        double fx = sscale * sum.total_x(), fy = sscale * sum.total_y();
         const R grav = (R) G.gravity;
         if (grav != (R) 0) {
             const R dist = fsqrt_r(me.x * me.x + me.y * me.y);
@@ -1239,9 +1292,10 @@ k_repulse(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restric
 
 // One step of a wave's walk (the body of k_repulse's walk, without the
 // counters) over the record `node`; returns the next record, -1 at the end.
-template <typename R, bool ANY_ADJUST>
+template <typename R, bool FP64_SUMS>
 __device__ __forceinline__ int walk_step(const TNode<R> &node, typename Vec<R>::v4 me,
-                                         R theta2, bool adjust, int &done, ForceSum<R> &sum,
+                                         R theta2, bool adjust, bool wide, int &done,
+                                         LaneForce<R, FP64_SUMS> &sum,
                                          const typename Vec<R>::v4 *__restrict__ psort)
 {
     using V4 = typename Vec<R>::v4;
@@ -1255,7 +1309,7 @@ __device__ __forceinline__ int walk_step(const TNode<R> &node, typename Vec<R>::
     const uint64_t fm = __builtin_amdgcn_ballot_w64(d2 * theta2 > rc.w);
     if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {
         asm volatile("");
-        sum.add(xd, yd, rc.z * frcp_r(d2));
+        sum.term(xd, yd, rc.z * frcp_r(d2), wide);
         done = rl.w;
     }
     const uint64_t tm = am & ~fm;
@@ -1275,14 +1329,14 @@ __device__ __forceinline__ int walk_step(const TNode<R> &node, typename Vec<R>::
 #pragma unroll
         for (int u = 0; u < kLeafSize; u++) {
             if (j0 + u > j1) break;
-            if (ANY_ADJUST && adjust) {
-                lane_pair_adjust<R>(me, o[u], sum, true);
+            if (FP64_SUMS && adjust) {
+                sum.pair_sized(me, o[u], true);
             } else {
                 const R pxd = me.x - o[u].x, pyd = me.y - o[u].y;
                 const R pd2 = pxd * pxd + pyd * pyd;
                 R w = pd2 > (R) 0 ? o[u].z * frcp_r(pd2) : (R) 0;
                 if (sizeof(R) == 4) asm volatile("" : "+v"(w));
-                sum.add(pxd, pyd, w);
+                sum.term(pxd, pyd, w, wide);
             }
         }
         done = j1;
@@ -1292,13 +1346,13 @@ __device__ __forceinline__ int walk_step(const TNode<R> &node, typename Vec<R>::
 
 // The force that leaves the traversal: repulsion minus gravity
 // (pipeline.cl:404-411), written in the original order.
-template <typename R>
+template <typename R, bool FP64>
 __device__ __forceinline__ void repulse_out(const GraphDesc &G, typename Vec<R>::v4 me,
-                                            const ForceSum<R> &sum, uint32_t dst,
+                                            const LaneForce<R, FP64> &sum, uint32_t dst,
                                             double2 *__restrict__ F)
 {
     const double sscale = G.scaling * (double) me.z;
-    double fx = sscale * sum.x, fy = sscale * sum.y;
+    double fx = sscale * sum.total_x(), fy = sscale * sum.total_y();
     const R grav = (R) G.gravity;
     if (grav != (R) 0) {
         const R dist = fsqrt_r(me.x * me.x + me.y * me.y);
@@ -1318,8 +1372,10 @@ __device__ __forceinline__ void repulse_out(const GraphDesc &G, typename Vec<R>:
 // either walk's arithmetic covers the other's wait. Each lane's terms, and
 // their order, are those of k_repulse. A wave whose 128 nodes do not all
 // belong to one graph walks its two halves one after the other, graph by
-// graph, as k_repulse does.
-template <typename R, bool ANY_ADJUST, bool WIDE>
+// graph, as k_repulse does. (Three or four walks per wave, with every record
+// load issued before any step, measured slower at 2^21 nodes: 0.76 and
+// 0.83 ms against 0.68 ms for two walks in that form.)
+template <typename R, bool FP64_SUMS, bool WIDE>
 __global__ void __launch_bounds__(1024)
 k_repulse2(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restrict__ gd,
            const GraphState *__restrict__ gs, const typename Vec<R>::v4 *__restrict__ psort,
@@ -1339,7 +1395,7 @@ k_repulse2(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restri
     int kk[2], gg[2];
     bool vv[2];
     V4 me[2];
-    ForceSum<R> sum[2];
+    LaneForce<R, FP64_SUMS> sum[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         kk[h] = kk0[h] < N ? kk0[h] : N - 1;
@@ -1354,7 +1410,8 @@ k_repulse2(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restri
     const bool one = __ballot((vv[0] && gg[0] != g) || (vv[1] && gg[1] != g)) == 0;
     if (one && any0 && any1) {
         const GraphDesc G = gd[g];
-        const bool adjust = ANY_ADJUST && (G.flags & GF_ADJUST);
+        const bool adjust = FP64_SUMS && (G.flags & GF_ADJUST);
+        const bool wide = FP64_SUMS && fp64_sums(G);
         const R theta2 = (R) G.theta2;
         int done0 = vv[0] ? INT_MIN + 1 : INT_MAX, done1 = vv[1] ? INT_MIN + 1 : INT_MAX;
         int c0 = G.ioff, c1 = G.ioff;
@@ -1364,16 +1421,16 @@ k_repulse2(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restri
         while (c0 >= 0 && c1 >= 0) {
             const TNode<R> n0 = record(c0), n1 = record(c1);
             c0 = __builtin_amdgcn_readfirstlane(
-                walk_step<R, ANY_ADJUST>(n0, me[0], theta2, adjust, done0, sum[0], psort));
+                walk_step<R, FP64_SUMS>(n0, me[0], theta2, adjust, wide, done0, sum[0], psort));
             c1 = __builtin_amdgcn_readfirstlane(
-                walk_step<R, ANY_ADJUST>(n1, me[1], theta2, adjust, done1, sum[1], psort));
+                walk_step<R, FP64_SUMS>(n1, me[1], theta2, adjust, wide, done1, sum[1], psort));
         }
         while (c0 >= 0)
-            c0 = __builtin_amdgcn_readfirstlane(walk_step<R, ANY_ADJUST>(
-                record(c0), me[0], theta2, adjust, done0, sum[0], psort));
+            c0 = __builtin_amdgcn_readfirstlane(walk_step<R, FP64_SUMS>(
+                record(c0), me[0], theta2, adjust, wide, done0, sum[0], psort));
         while (c1 >= 0)
-            c1 = __builtin_amdgcn_readfirstlane(walk_step<R, ANY_ADJUST>(
-                record(c1), me[1], theta2, adjust, done1, sum[1], psort));
+            c1 = __builtin_amdgcn_readfirstlane(walk_step<R, FP64_SUMS>(
+                record(c1), me[1], theta2, adjust, wide, done1, sum[1], psort));
     } else {
         // the halves one after the other, each graph by graph
 #pragma unroll
@@ -1385,19 +1442,20 @@ k_repulse2(int N, int NI, const int *__restrict__ gid, const GraphDesc *__restri
                 const bool mine = gg[h] == gh;
                 todo &= ~__ballot(mine);
                 const GraphDesc G = gd[gh];
-                const bool adjust = ANY_ADJUST && (G.flags & GF_ADJUST);
+                const bool adjust = FP64_SUMS && (G.flags & GF_ADJUST);
+                const bool wide = FP64_SUMS && fp64_sums(G);
                 const R theta2 = (R) G.theta2;
                 int done = mine ? INT_MIN + 1 : INT_MAX;
                 int c = G.ioff;
                 while (c >= 0)
-                    c = __builtin_amdgcn_readfirstlane(walk_step<R, ANY_ADJUST>(
-                        record(c), me[h], theta2, adjust, done, sum[h], psort));
+                    c = __builtin_amdgcn_readfirstlane(walk_step<R, FP64_SUMS>(
+                        record(c), me[h], theta2, adjust, wide, done, sum[h], psort));
             }
         }
     }
 #pragma unroll
     for (int h = 0; h < 2; h++)
-        if (vv[h]) repulse_out<R>(gd[gg[h]], me[h], sum[h], order[kk[h]], F);
+        if (vv[h]) repulse_out<R, FP64_SUMS>(gd[gg[h]], me[h], sum[h], order[kk[h]], F);
 }
 
 // ---------------------------------------------------------------------------
@@ -1864,7 +1922,7 @@ template <typename R> struct EngineImpl final : Engine {
     std::vector<int4> h_class[2];
     std::vector<int4> h_items, h_hubs;
     int nslots = 0;
-    bool any_adjust = false;
+    bool any_fp64_sums = false; // a graph with sizes or at theta 0: FP64 traversal sums
     int parity = 0; // F[parity] receives the force of the next iteration
 
     DevBuf<GraphDesc> d_gd;
@@ -1975,8 +2033,9 @@ template <typename R> struct EngineImpl final : Engine {
     void launch_attraction(hipStream_t st);
     void drop_graph();
 
-    // the repulsion kernel for this batch: with sizes or not, counting or
-    // not, record offsets in 32 or 64 bits; `threads` per workgroup (64 to
+    // the repulsion kernel for this batch: with FP64 sums (some graph with
+    // sizes or at theta 0) or not, counting or not, record offsets in 32 or
+    // 64 bits; `threads` per workgroup (64 to
     // 1024) and XCD runs of `xrun` blocks (0: none); each wave walks for the
     // same 64 nodes whatever the shape
     template <bool COUNT> void launch_repulse(double2 *Fout, unsigned long long *cnt,
@@ -1995,7 +2054,7 @@ template <typename R> struct EngineImpl final : Engine {
     else                                                                                         \
         k_repulse2<R, ADJ, WIDE><<<blocks, threads, 0, stream>>>(                               \
             N, NI, d_gid.p, d_gd.p, d_gs.p, d_psort.p, d_tn.p, d_valA.p, Fout, xrun)
-        if (any_adjust) {
+        if (any_fp64_sums) {
             if (wide) FA2_REPULSE(true, true);
             else FA2_REPULSE(true, false);
         } else {
@@ -2054,7 +2113,7 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         D.gravity = p->gravity;
         D.theta2 = p->theta * p->theta;
         D.jt = p->jitter_tolerance;
-        any_adjust = any_adjust || p->adjust;
+        any_fp64_sums = any_fp64_sums || p->adjust || D.theta2 == 0.0;
         for (int j = 0; j < D.ntiles; j++) {
             Tile t;
             t.g = g;
@@ -3073,7 +3132,7 @@ template <typename R> int EngineImpl<R>::set_tuning(const fa2_tuning *t)
 template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, double *times)
 {
     (void) hipSetDevice(device);
-    const TuneKey key{device, (int) sizeof(R), any_adjust ? 1 : 0, magnitude(N),
+    const TuneKey key{device, (int) sizeof(R), any_fp64_sums ? 1 : 0, magnitude(N),
                       magnitude((long long) h_class[0].size()),
                       magnitude((long long) h_class[1].size()), magnitude((long long) h_items.size())};
     if (!force) {

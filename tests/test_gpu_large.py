@@ -346,6 +346,37 @@ def test_reproducible_and_same_alone_as_in_a_batch():
             assert np.array_equal(one.positions(0), together[i])
 
 
+def test_graphs_without_fp64_sums_are_unchanged_beside_those_with():
+    """A batch holding a graph at theta 0 or with node sizes runs the
+    traversal kernel that carries FP64 force sums; the graphs that keep
+    FP32 sums (theta > 0, no sizes) give the same layout bit for bit there
+    as alone, and so do the others. The theta-0 graph's forces are within
+    1e-5 normwise of the oracle (its lanes sum every other node)."""
+    specs = [(1500, dict(linlog=True)), (900, dict(theta=0.0)),
+             (1200, dict(adjust=True, size=1.5)), (2100, dict(theta=0.8))]
+    gs = []
+    for i, (m, kw) in enumerate(specs):
+        m, ee = pa(m, 2, seed=40 + i)
+        gs.append(dict(n=m, edges=ee, init=start(m, seed=50 + i), iterations=12, **kw))
+    with Batch(gs, autotune=False) as bb:
+        bb.run(12)
+        bb.sync()
+        together = [bb.positions(i) for i in range(len(gs))]
+    for i, g in enumerate(gs):
+        with Batch([g], autotune=False) as one:
+            one.run(12)
+            one.sync()
+            assert np.array_equal(one.positions(0), together[i]), specs[i]
+    g = gs[1]
+    with Batch([dict(g, iterations=1)], autotune=False) as one:
+        one.run(1)
+        one.sync()
+        f = one.forces(0)
+    _, tr = O.large_run(g["n"], g["edges"], g["init"], 1, mode="m", theta=0.0, trace=True)
+    ref = tr["force"][0]
+    assert np.abs(f - ref).max() / np.abs(ref).max() <= 1e-5
+
+
 def test_layout_quality_matches_cpu():
     """test-large.R:73-94 (5%) and the 2% edge-length/stress gate: islands
     4 x 100, 300 iterations."""
