@@ -166,8 +166,8 @@ def cpu_baseline(n, e, pos, args):
 # incidence for the attraction's list entry and position gather: 456n + 16I
 # in total. FP64 position accumulators add 16 bytes per node to the move.
 STAGE_BYTES_PER_NODE = dict(codes=8 + 16,            # bbox, Morton codes
-                            sort=4 * 16 + 32,         # 4 LSD passes, payload gather
-                            tree=32 + 144 + 40,       # Karras, summaries, finalize
+                            sort=4 * 16,              # 4 LSD passes
+                            tree=32 + 32 + 144 + 40,  # payload gather, Karras, summaries, finalize
                             repulsion=28,             # streaming part of the traversal
                             attraction=32,            # per node; + 16 per incidence
                             move=24 + 36)             # swing, apply

@@ -344,17 +344,15 @@ k_sort_rowscan_block(const int *__restrict__ glist, const GraphDesc *__restrict_
 // Stable scatter of one tile. Wave w ranks its 512 keys in 8 rounds of 64
 // (ballot match on the 8 digit bits), then the waves are combined in order:
 // the rank of a key counts the equal digits before it in the tile. Keys go
-// through LDS so that the global writes of one digit are contiguous.
-// The last pass (GATHER) also writes psort[k] = node4[order[k]], the
-// (x, y, mass, radius) of the nodes in sorted order (pipeline.cl:195-207).
-template <typename V4, bool GATHER>
+// through LDS so that the global writes of one digit are contiguous. (The
+// node records in sorted order, pipeline.cl:195-207, are gathered by
+// k_karras, whose searches hide the random reads.)
 __global__ void __launch_bounds__(kSortThreads)
 k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
                  const GraphState *__restrict__ gs, const uint32_t *__restrict__ key_in,
                  const uint32_t *__restrict__ val_in,
                  int shift, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ rowtot,
-                 uint32_t *__restrict__ key_out, uint32_t *__restrict__ val_out,
-                 const V4 *__restrict__ node4, V4 *__restrict__ psort)
+                 uint32_t *__restrict__ key_out, uint32_t *__restrict__ val_out)
 {
     __shared__ uint32_t wcnt[4][256];
     __shared__ int goff[256];
@@ -453,10 +451,7 @@ k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
         sval[p] = vv[r];
     }
     __syncthreads();
-    // each thread moves kSortItems keys; with GATHER, the (x, y, mass,
-    // radius) loads of all of them are issued before the first store, so
-    // that their latencies overlap (a load past the tile's end reads node 0
-    // and is not stored)
+    // each thread moves kSortItems keys
     uint32_t ko[kSortItems], vo[kSortItems];
     int dsto[kSortItems];
 #pragma unroll
@@ -467,23 +462,12 @@ k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
         vo[r] = in ? sval[p] : 0u;
         dsto[r] = in ? goff[(ko[r] >> shift) & 255u] + p : -1;
     }
-    V4 nv[GATHER ? kSortItems : 1];
-    if (GATHER) {
-#pragma unroll
-        for (int r = 0; r < kSortItems; r++) nv[r] = node4[vo[r]];
-        // all loads complete here, before any store: the compiler would
-        // otherwise sink each load to its store and wait for one at a time
-#pragma unroll
-        for (int r = 0; r < kSortItems; r++)
-            asm volatile("" : "+v"(nv[r].x), "+v"(nv[r].y), "+v"(nv[r].z), "+v"(nv[r].w));
-    }
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
         const int dst = dsto[r];
         if (dst < G.noff || dst >= G.noff + G.n) continue; // past the tile's end only
         key_out[dst] = ko[r];
         val_out[dst] = vo[r];
-        if (GATHER) psort[dst] = nv[r];
     }
 }
 
@@ -544,9 +528,9 @@ int radix_sort_pairs_u32(hipStream_t st, hipStream_t ast, int n, int bits, uint3
         else
             k_sort_rowscan_block<<<256, 256, 0, st>>>(d_glist.p, d_gd.p, d_gs.p, d_hist.p,
                                                       d_rowtot.p);
-        k_sort_downsweep<float4, false><<<T, kSortThreads, 0, st>>>(
-            d_tiles.p, d_gd.p, d_gs.p, k[in], v[in], 8 * pass, d_hist.p, d_rowtot.p, k[in ^ 1],
-            v[in ^ 1], nullptr, nullptr);
+        k_sort_downsweep<<<T, kSortThreads, 0, st>>>(d_tiles.p, d_gd.p, d_gs.p, k[in], v[in],
+                                                      8 * pass, d_hist.p, d_rowtot.p, k[in ^ 1],
+                                                      v[in ^ 1]);
         hs.check(hipGetLastError(), "launch");
         in ^= 1;
     }
@@ -636,7 +620,9 @@ k_karras(int NI, int N, const int *__restrict__ gi, const GraphDesc *__restrict_
          const GraphState *__restrict__ gs, const uint32_t *__restrict__ ks,
          int2 *__restrict__ child, TNode<R> *__restrict__ tn, int *__restrict__ parent_leaf,
          int *__restrict__ parent_int, int *__restrict__ escape, const int *__restrict__ ubase,
-         int *__restrict__ ucnt, int *__restrict__ lists, int *__restrict__ xslot)
+         int *__restrict__ ucnt, int *__restrict__ lists, int *__restrict__ xslot,
+         const uint32_t *__restrict__ order, const typename Vec<R>::v4 *__restrict__ node4,
+         typename Vec<R>::v4 *__restrict__ psort)
 {
     __shared__ uint32_t s_key[kKarrasStage];
     const int t0 = (int) blockIdx.x * kKarrasThreads;
@@ -646,6 +632,13 @@ k_karras(int NI, int N, const int *__restrict__ gi, const GraphDesc *__restrict_
     const int g = gi[min(t, NI - 1)];
     const bool retired = gs[g].retired;
     const GraphDesc G = gd[g];
+    // The gather of the node records in sorted order (pipeline.cl:195-207):
+    // psort[p] = node4[order[p]] at the position p = t + g of region t (the
+    // graph's last position, which has no region, by its last region). The
+    // random reads are issued first and their records stored last, so that
+    // they wait beside the key staging and the searches.
+    const int pg = min(t, NI - 1) + g;
+    const typename Vec<R>::v4 nv = node4[order[pg]];
     const int lo = t0 + gi[t0] - kKarrasWin; // s_key[q] holds the key at position lo + q
     const int cnt = min(tl + gi[tl] + kKarrasWin + 1 - lo, kKarrasStage);
     constexpr int kPer = kKarrasStage / kKarrasThreads;
@@ -718,6 +711,8 @@ k_karras(int NI, int N, const int *__restrict__ gi, const GraphDesc *__restrict_
         }
     }
     xslot[t] = xs;
+    psort[pg] = nv;
+    if (i == n - 2) psort[base + n - 1] = node4[order[base + n - 1]];
 }
 
 template <typename R> struct Summ {
@@ -2619,7 +2614,7 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
     k_morton<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gs.p, d_pos.p, d_gbox.p, d_keyA.p, d_valA.p,
                                        (int) h_units.size(), d_ucnt.p);
     mark(ev);
-    // 3: sort, A -> B -> A -> B -> A; the last pass gathers psort
+    // 3: sort, A -> B -> A -> B -> A (k_karras gathers psort)
     for (int pass = 0; pass < 4; pass++) {
         uint32_t *ki = pass % 2 ? d_keyB.p : d_keyA.p, *vi = pass % 2 ? d_valB.p : d_valA.p;
         uint32_t *ko = pass % 2 ? d_keyA.p : d_keyB.p, *vo = pass % 2 ? d_valA.p : d_valB.p;
@@ -2634,14 +2629,8 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
         if (const int n2 = (int) h_rowg[2].size())
             k_sort_rowscan_block<<<256 * n2, 256, 0, stream>>>(d_rowg[2].p, d_gd.p, d_gs.p,
                                                                d_hist.p, d_rowtot.p);
-        if (pass < 3)
-            k_sort_downsweep<V4, false><<<T, kSortThreads, 0, stream>>>(
-                d_tiles.p, d_gd.p, d_gs.p, ki, vi, 8 * pass, d_hist.p, d_rowtot.p, ko, vo, nullptr,
-                nullptr);
-        else
-            k_sort_downsweep<V4, true><<<T, kSortThreads, 0, stream>>>(
-                d_tiles.p, d_gd.p, d_gs.p, ki, vi, 8 * pass, d_hist.p, d_rowtot.p, ko, vo, d_node4.p,
-                d_psort.p);
+        k_sort_downsweep<<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, ki, vi, 8 * pass,
+                                                          d_hist.p, d_rowtot.p, ko, vo);
     }
     mark(ev);
     if (fork == 1) fork_attraction();
@@ -2649,7 +2638,7 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
     if (NI > 0) {
         k_karras<R><<<ceil_div(NI, kKarrasThreads), kKarrasThreads, 0, stream>>>(
             NI, N, d_gi.p, d_gd.p, d_gs.p, d_keyA.p, d_child.p, d_tn.p, d_parent_leaf.p, d_parent_int.p,
-            d_escape.p, d_ubase.p, d_ucnt.p, d_lists.p, d_xslot.p);
+            d_escape.p, d_ubase.p, d_ucnt.p, d_lists.p, d_xslot.p, d_valA.p, d_node4.p, d_psort.p);
         k_aggregate_tile<R><<<T * (kTile / agg_block<R>()), 1024, 0, stream>>>(
             d_tiles.p, d_gd.p, d_gs.p, d_psort.p, d_child.p, d_parent_leaf.p, d_parent_int.p,
             d_escape.p, NI, d_nd.p, d_tn.p);
