@@ -104,8 +104,7 @@ def sort_yardstick(keys, engine_ms, reps=50):
     """rocPRIM's radix_sort_pairs (libfa2yard.so, built beside the library
     and never part of it) on the same 2^21 (Morton code, node) pairs as the
     engine's sort, all 32 bits: a stated yardstick for the engine's sort
-    phase, which also keeps each graph of a batch apart and gathers the
-    sorted node records in its last pass."""
+    phase, which also keeps each graph of a batch apart."""
     import ctypes
     path = os.path.join(ROOT, "forceatlas2_rc_amd", "libfa2yard.so")
     if not os.path.exists(path):
