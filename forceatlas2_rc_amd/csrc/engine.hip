@@ -1560,6 +1560,20 @@ template <int GL> __device__ inline void group_sum(double &x, double &y)
 // changes.
 constexpr int kAttractXcdRun = 32;
 
+// While the attraction runs beside another phase (forked onto the second
+// stream, beside the traversal by default), each of its workgroups reserves
+// 54 KB of LDS that it does not use: a CU (160 KB) then holds at most two of
+// them, eight of its 32 wave slots, and the traversal keeps the rest. The
+// attraction alone gets slower (0.274 -> 0.309 ms on the 2^21-node R-MAT
+// graph), but it still ends well before the traversal, which it now slows
+// less: the replayed iteration went from 1.073 to 1.061 ms there, and from
+// 0.969 to 0.961 ms on the batch of 512 block-model graphs (two interleaved
+// pairs of runs each; the tuner times the attraction's unrolls with the
+// reservation too). Marking the position gathers non-temporal instead, so
+// that they would not evict the traversal's records from L2, made the
+// iteration 1.17 ms: the gathers reuse the lines of the hubs' positions.
+constexpr unsigned kAttractBesideLds = 54 * 1024;
+
 // GL lanes per node; the work list holds (node, first incidence, end, graph).
 // The one-lane class (GL = 1) reads its lists in the sliced layout: the 64
 // lanes of a wave have their r-th entries side by side (entry r of lane l at
@@ -2023,9 +2037,9 @@ template <typename R> struct EngineImpl final : Engine {
     void enqueue_forces(double2 *Fout, int fork);
     void fork_attraction();
     // the attraction kernel of class c at unroll u, writing into Fa
-    void launch_attract_class(int c, int u, double2 *Fa, hipStream_t st);
+    void launch_attract_class(int c, int u, double2 *Fa, hipStream_t st, bool beside = false);
     // every attraction kernel of an iteration, on st
-    void launch_attraction(hipStream_t st);
+    void launch_attraction(hipStream_t st, bool beside = false);
     void drop_graph();
 
     // the repulsion kernel for this batch: with FP64 sums (some graph with
@@ -2518,21 +2532,22 @@ template <typename R> void EngineImpl<R>::mark(std::vector<hipEvent_t> *ev)
 }
 
 template <typename R>
-void EngineImpl<R>::launch_attract_class(int c, int u, double2 *Fa, hipStream_t st)
+void EngineImpl<R>::launch_attract_class(int c, int u, double2 *Fa, hipStream_t st, bool beside)
 {
     const int cnt = c < 2 ? (int) h_class[c].size() : (int) h_items.size();
     if (cnt == 0) return;
+    const unsigned lds = beside ? kAttractBesideLds : 0;
 #define FA2_ATTRACT(U)                                                                          \
     do {                                                                                        \
         if (c == 0)                                                                             \
-            k_attract<R, 1, U><<<ceil_div(cnt, 256), 256, 0, st>>>(                         \
+            k_attract<R, 1, U><<<ceil_div(cnt, 256), 256, lds, st>>>(                         \
                 cnt, d_class[0].p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_sell_node.p,            \
                 d_sell_coef.p, Fa);                                                             \
         else if (c == 1)                                                                        \
-            k_attract<R, 16, U><<<ceil_div(cnt, 16), 256, 0, st>>>(                         \
+            k_attract<R, 16, U><<<ceil_div(cnt, 16), 256, lds, st>>>(                         \
                 cnt, d_class[1].p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_node.p, d_coef.p, Fa); \
         else                                                                                    \
-            k_attract_items<R, U><<<ceil_div(cnt, 4), 256, 0, st>>>(                        \
+            k_attract_items<R, U><<<ceil_div(cnt, 4), 256, lds, st>>>(                        \
                 cnt, d_items.p, d_gid.p, d_gd.p, d_gs.p, d_pos.p, d_size.p, d_inc_node.p,        \
                 d_coef.p, Fa, d_hubpart.p);                                                     \
     } while (0)
@@ -2545,13 +2560,14 @@ void EngineImpl<R>::launch_attract_class(int c, int u, double2 *Fa, hipStream_t 
 #undef FA2_ATTRACT
 }
 
-template <typename R> void EngineImpl<R>::launch_attraction(hipStream_t st)
+template <typename R> void EngineImpl<R>::launch_attraction(hipStream_t st, bool beside)
 {
     // the hubs' work items first: the largest kernel gets the wave slots that
     // free up first beside the traversal, and the lists of few incidences
     // fill in after it (each kernel writes its own nodes: the order changes
     // no result)
-    for (int c = 2; c >= 0; c--) launch_attract_class(c, shape.attract_unroll[c], d_Fa.p, st);
+    for (int c = 2; c >= 0; c--)
+        launch_attract_class(c, shape.attract_unroll[c], d_Fa.p, st, beside);
     if (!h_hubs.empty())
         k_attract_fix<R><<<ceil_div((int) h_hubs.size(), 4), 256, 0, st>>>(
             (int) h_hubs.size(), d_hubs.p, d_gid.p, d_gs.p, d_hubpart.p, d_Fa.p);
@@ -2561,7 +2577,7 @@ template <typename R> void EngineImpl<R>::fork_attraction()
 {
     (void) hipEventRecord(ev_fork, stream);
     (void) hipStreamWaitEvent(s_att, ev_fork, 0);
-    launch_attraction(s_att);
+    launch_attraction(s_att, true);
     (void) hipEventRecord(ev_join, s_att);
 }
 
@@ -3191,7 +3207,8 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
             [&] { launch_repulse<false>(Fs, nullptr, pick.repulse_threads, runs[k]); });
     pick.repulse_xcd_run = runs[choose(t + 5, 5, 2)];
     // an attraction kernel runs after the traversal, whose records and
-    // nodes fill the caches: each candidate is timed right after one
+    // nodes fill the caches: each candidate is timed right after one, with
+    // the LDS reservation it runs with beside the traversal
     for (int c = 0; c < 3; c++) {
         const size_t cnt = c < 2 ? h_class[c].size() : h_items.size();
         if (cnt == 0) continue;
@@ -3200,7 +3217,7 @@ template <typename R> int EngineImpl<R>::tune(int force, fa2_tuning *chosen, dou
             for (int rep = 0; rep < 3 && err == hipSuccess; rep++) {
                 launch_repulse<false>(Fs, nullptr, pick.repulse_threads, pick.repulse_xcd_run);
                 const double tk =
-                    best_of_1([&] { launch_attract_class(c, unroll[k], d_Fa.p, stream); });
+                    best_of_1([&] { launch_attract_class(c, unroll[k], d_Fa.p, stream, true); });
                 if (rep == 0 || tk < best) best = tk;
             }
             t[10 + 4 * c + k] = best;
