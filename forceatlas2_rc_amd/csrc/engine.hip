@@ -182,30 +182,43 @@ __device__ inline double div_rn(double a, double b) { return __ddiv_rn(a, b); }
 // Morton code of every node: 16 bits per axis, x in the even bits, computed
 // with IEEE arithmetic as written (the codes must match the reference bit for
 // bit, pipeline.cl:116-130).
+//
+// The tile's counts of the sort's first digit (the low 8 bits) are made here
+// too, as k_sort_upsweep makes them for the later passes, so the first pass
+// starts at its row scans.
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_morton(const Tile *__restrict__ tiles, const GraphState *__restrict__ gs,
-         const typename Vec<R>::v2 *__restrict__ pos, const typename Vec<R>::v4 *__restrict__ gbox,
-         uint32_t *__restrict__ key, uint32_t *__restrict__ val, int UT, int *__restrict__ ucnt)
+k_morton(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
+         const GraphState *__restrict__ gs, const typename Vec<R>::v2 *__restrict__ pos,
+         const typename Vec<R>::v4 *__restrict__ gbox, uint32_t *__restrict__ key,
+         uint32_t *__restrict__ val, uint32_t *__restrict__ hist, int UT, int *__restrict__ ucnt)
 {
+    __shared__ uint32_t h[256]; // one digit's count per thread (256 threads)
     // the lengths of the summary unit lists, filled by k_karras
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < UT; q += gridDim.x * blockDim.x)
         ucnt[q * kCountStride] = 0;
     const Tile t = tiles[blockIdx.x];
     if (gs[t.g].retired) return; // its tree is not used any more
+    h[threadIdx.x] = 0;
     const typename Vec<R>::v4 bb = gbox[t.g];
     const R ex = bb.z - bb.x, ey = bb.w - bb.y;
     const R extent = ex > ey ? ex : ey;
     const R scale = extent > (R) 0 ? div_rn((R) 65535, extent) : (R) 0;
+    __syncthreads();
     for (int i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
         const typename Vec<R>::v2 p = pos[i];
         R fx = (p.x - bb.x) * scale;
         R fy = (p.y - bb.y) * scale;
         fx = min(max(fx, (R) 0), (R) 65535);
         fy = min(max(fy, (R) 0), (R) 65535);
-        key[i] = spread16((uint32_t) fx) | (spread16((uint32_t) fy) << 1);
+        const uint32_t k = spread16((uint32_t) fx) | (spread16((uint32_t) fy) << 1);
+        key[i] = k;
         val[i] = (uint32_t) i;
+        atomicAdd(&h[k & 255u], 1u);
     }
+    __syncthreads();
+    const GraphDesc g = gd[t.g];
+    hist[(size_t) g.toff * 256 + (size_t) threadIdx.x * g.ntiles + t.j] = h[threadIdx.x];
 }
 
 // ---------------------------------------------------------------------------
@@ -2627,15 +2640,16 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
     mark(ev);
     // 1-2: codes
     k_graph_bbox<R><<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_gs.p, d_tbox.p, d_gbox.p);
-    k_morton<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gs.p, d_pos.p, d_gbox.p, d_keyA.p, d_valA.p,
-                                       (int) h_units.size(), d_ucnt.p);
+    k_morton<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, d_pos.p, d_gbox.p, d_keyA.p,
+                                       d_valA.p, d_hist.p, (int) h_units.size(), d_ucnt.p);
     mark(ev);
     // 3: sort, A -> B -> A -> B -> A (k_karras gathers psort)
     for (int pass = 0; pass < 4; pass++) {
         uint32_t *ki = pass % 2 ? d_keyB.p : d_keyA.p, *vi = pass % 2 ? d_valB.p : d_valA.p;
         uint32_t *ko = pass % 2 ? d_keyA.p : d_keyB.p, *vo = pass % 2 ? d_valA.p : d_valB.p;
-        k_sort_upsweep<<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, ki, 8 * pass,
-                                                        d_hist.p);
+        if (pass > 0) // the first pass's counts come from k_morton
+            k_sort_upsweep<<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, ki, 8 * pass,
+                                                            d_hist.p);
         if (const int n0 = (int) h_rowg[0].size())
             k_sort_rowscan_thread<<<n0, 256, 0, stream>>>(256 * n0, d_rowg[0].p, d_gd.p, d_gs.p,
                                                           d_hist.p, d_rowtot.p);
