@@ -152,12 +152,19 @@ k_graph_bbox(int G, const GraphDesc *__restrict__ gd, const GraphState *__restri
     const int lane = lane_id();
     const int t0 = gd[g].toff, nt = gd[g].ntiles;
     R a = (R) INFINITY, b = (R) INFINITY, c = (R) -INFINITY, d = (R) -INFINITY;
-    for (int t = lane; t < nt; t += 64) {
-        const typename Vec<R>::v4 v = tbox[t0 + t];
-        a = min(a, v.x);
-        b = min(b, v.y);
-        c = max(c, v.z);
-        d = max(d, v.w);
+    // eight boxes per lane in flight at once (a slot past the end reads the
+    // last box again, which changes no minimum or maximum)
+    for (int t = lane; t < nt; t += 64 * 8) {
+        typename Vec<R>::v4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = tbox[t0 + min(t + 64 * u, nt - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            a = min(a, v[u].x);
+            b = min(b, v[u].y);
+            c = max(c, v[u].z);
+            d = max(d, v[u].w);
+        }
     }
     a = wave_min(a);
     b = wave_min(b);
@@ -797,6 +804,7 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     __shared__ int2 s_child[B];
     __shared__ int s_parent[B];
     __shared__ int s_vis[B];
+    __shared__ int s_span[B]; // last - first of each region inside
     __shared__ R s_sum[7][B]; // x0, y0, x1, y1, m, mx, my
     const Tile T = tiles[blockIdx.x / kParts];
     const int g = T.g;
@@ -837,21 +845,32 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     const int esc_last = inside ? escape[min(l.w, gend - 2) - g] : 0;
     if (has_leaf) {
         s_leaf[q] = leaf;
-        // the leaf's own traversal record (NI + sorted position): a region of
-        // one node and size 0. Accepted as a monopole it gives exactly the
-        // pair term without sizes; with sizes it is never accepted (size^2
-        // infinite) and is visited as a pair.
-        TNode<R> rec;
-        rec.c = mk4<R>(leaf.x, leaf.y, leaf.z, adjust ? (R) INFINITY : (R) 0);
-        rec.l = make_int4(0, pos == gend - 1 ? kNoRecord : record_of(esc_pos, NI), pos, pos);
-        tn[(size_t) NI + pos] = rec;
         if (has_slot) {
             s_child[q] = ch;
             s_parent[q] = par;
+            s_span[q] = l.w - l.z;
         }
         s_vis[q] = inside ? 0 : kOutside;
     }
     __syncthreads();
+    // the leaf's own traversal record (NI + sorted position): a region of one
+    // node and size 0. Accepted as a monopole it gives exactly the pair term
+    // without sizes; with sizes it is never accepted (size^2 infinite) and is
+    // visited as a pair. A walk reaches a leaf's record only inside its
+    // parent, and enters a region only when it has more than kLeafSize nodes
+    // (a smaller one is visited node by node from psort and left by its skip
+    // link): the record of a leaf whose parent is such a small region is
+    // never read, and is not written (most leaves; a parent outside the
+    // block counts as large).
+    if (has_leaf) {
+        const int pq = pl - tbase;
+        if (!(pq >= 0 && pq < nslots && s_span[pq] < kLeafSize)) {
+            TNode<R> rec;
+            rec.c = mk4<R>(leaf.x, leaf.y, leaf.z, adjust ? (R) INFINITY : (R) 0);
+            rec.l = make_int4(0, pos == gend - 1 ? kNoRecord : record_of(esc_pos, NI), pos, pos);
+            tn[(size_t) NI + pos] = rec;
+        }
+    }
     if (has_leaf) {
         int t = pl;
         Summ<R> r = leaf_summ<R>(leaf);
@@ -1795,10 +1814,19 @@ k_speed(int G, const GraphDesc *__restrict__ gd, const double2 *__restrict__ par
     const int lane = lane_id();
     const GraphDesc D = gd[g];
     double a = 0.0, b = 0.0;
-    for (int t = lane; t < D.ntiles; t += 64) {
-        const double2 v = part[D.toff + t];
-        a += v.x;
-        b += v.y;
+    // eight partials per lane in flight at once, added in the order of the
+    // one-at-a-time loop (tile lane, lane + 64, ...)
+    for (int t = lane; t < D.ntiles; t += 64 * 8) {
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = part[D.toff + min(t + 64 * u, D.ntiles - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (t + 64 * u < D.ntiles) {
+                a += v[u].x;
+                b += v[u].y;
+            }
+        }
     }
     a = wave_sum(a);
     b = wave_sum(b);

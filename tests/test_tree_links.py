@@ -112,6 +112,10 @@ def stackless_visits(T, g, k, accept):
             cur = skip
         else:
             p = ~cur
+            # k_aggregate_tile writes a leaf's record only when the leaf's
+            # parent holds more than 8 nodes: a walk never reaches another
+            par = T["tl"][T["parent_leaf"][p]]
+            assert par[3] - par[2] >= 8, "a leaf record under a small region"
             out.append(("l", p - base))
             cur = END if p == gend - 1 else T["escape"][p - g]
     assert cur == END, "the walk did not end"
@@ -214,6 +218,8 @@ def wave_visits(T, g, lanes, accept):
             c = skip
         else:
             p = ~c
+            par = T["tl"][T["parent_leaf"][p]]
+            assert par[3] - par[2] >= 8, "a leaf record under a small region"
             for k in lanes:
                 if p > done[k]:
                     seq[k].append(("l", p - base))
