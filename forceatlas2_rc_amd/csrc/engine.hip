@@ -340,23 +340,35 @@ k_sort_rowscan_block(const int *__restrict__ glist, const GraphDesc *__restrict_
     uint32_t *row = hist + (size_t) G.toff * 256 + (size_t) d * G.ntiles;
     const int lane = lane_id(), w = threadIdx.x >> 6;
     uint32_t carry = 0;
-    for (int b = 0; b < G.ntiles; b += 256) {
-        const int j = b + (int) threadIdx.x;
-        const uint32_t v = j < G.ntiles ? row[j] : 0u;
-        uint32_t s = v;
+    // chunks of 256 tiles, kRowChunks of them loaded at once (one round trip
+    // for rows of up to 2048 tiles), then scanned one after the other
+    constexpr int kRowChunks = 8;
+    for (int b0 = 0; b0 < G.ntiles; b0 += 256 * kRowChunks) {
+        uint32_t v[kRowChunks];
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t u = __shfl_up(s, o, 64);
-            if (lane >= o) s += u;
+        for (int c = 0; c < kRowChunks; c++) {
+            const int j = b0 + 256 * c + (int) threadIdx.x;
+            v[c] = j < G.ntiles ? row[j] : 0u;
         }
-        if (lane == 63) wsum[w] = s;
-        __syncthreads();
-        uint32_t before = carry;
-        for (int k = 0; k < w; k++) before += wsum[k];
-        const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-        if (j < G.ntiles) row[j] = before + s - v;
-        __syncthreads();
-        carry += total;
+#pragma unroll
+        for (int c = 0; c < kRowChunks; c++) {
+            const int j = b0 + 256 * c + (int) threadIdx.x;
+            if (b0 + 256 * c >= G.ntiles) break; // uniform
+            uint32_t s = v[c];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t u = __shfl_up(s, o, 64);
+                if (lane >= o) s += u;
+            }
+            if (lane == 63) wsum[w] = s;
+            __syncthreads();
+            uint32_t before = carry;
+            for (int k = 0; k < w; k++) before += wsum[k];
+            const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+            if (j < G.ntiles) row[j] = before + s - v[c];
+            __syncthreads();
+            carry += total;
+        }
     }
     if (threadIdx.x == 0) rowtot[g * 256 + d] = carry;
 }
