@@ -18,11 +18,6 @@ void set_hip_error(const char *what, hipError_t e);
 // same way, src/gpu.c:236-321). Sets the device current. nullptr on error.
 hipStream_t device_stream(int device);
 
-// Second stream of a device (default priority), made with the first and on
-// another hardware queue: the stream the batches fork their attraction onto.
-// Kept until fa2_gpu_shutdown(). nullptr on error.
-hipStream_t device_side_stream(int device);
-
 // Number of devices (0 when the runtime is unusable), cached.
 int device_count();
 

@@ -32,7 +32,6 @@ static std::mutex g_mu;
 static int g_ndev = -1;
 static constexpr int kMaxDevices = 64;
 static hipStream_t g_stream[kMaxDevices] = {};
-static hipStream_t g_side[kMaxDevices] = {};
 
 int device_count()
 {
@@ -62,30 +61,15 @@ hipStream_t device_stream(int device)
     }
     if (!g_stream[device]) {
         // default priority on purpose: streams of different priorities are
-        // never mixed in this library. The side stream is made right after
-        // the main one, so that the runtime, which hands its hardware queues
-        // to streams in turn as they are created, gives the two different
-        // queues; made later (per batch, after a server had made and
-        // destroyed its streams), it shared the main stream's queue, and the
-        // attraction forked onto it ran after the traversal instead of beside
-        // it (single layouts after a server 46% slower with 16 queues).
+        // never mixed in this library
         e = hipStreamCreateWithFlags(&g_stream[device], hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&g_side[device], hipStreamNonBlocking);
         if (e != hipSuccess) {
-            if (g_stream[device]) (void) hipStreamDestroy(g_stream[device]);
-            g_stream[device] = g_side[device] = nullptr;
+            g_stream[device] = nullptr;
             set_hip_error("could not create a HIP stream", e);
             return nullptr;
         }
     }
     return g_stream[device];
-}
-
-hipStream_t device_side_stream(int device)
-{
-    if (!device_stream(device)) return nullptr;
-    std::lock_guard<std::mutex> lk(g_mu);
-    return g_side[device];
 }
 
 } // namespace fa2
@@ -126,10 +110,8 @@ extern "C" void fa2_gpu_shutdown(void)
         if (g_stream[d]) {
             (void) hipSetDevice(d);
             (void) hipStreamSynchronize(g_stream[d]);
-            (void) hipStreamSynchronize(g_side[d]);
             (void) hipStreamDestroy(g_stream[d]);
-            (void) hipStreamDestroy(g_side[d]);
-            g_stream[d] = g_side[d] = nullptr;
+            g_stream[d] = nullptr;
         }
     }
     g_ndev = -1;

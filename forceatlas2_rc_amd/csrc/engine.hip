@@ -2001,7 +2001,7 @@ template <typename R> struct EngineImpl final : Engine {
     // the attraction of the iteration, computed beside the repulsion on a
     // second stream and added to it by k_swing (zero for nodes without edges)
     DevBuf<double2> d_Fa;
-    hipStream_t s_att = nullptr; // the device's side stream (not owned)
+    hipStream_t s_att = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DevBuf<double2> d_acc;
     DevBuf<R> d_mass, d_size, d_coef;
@@ -2049,6 +2049,7 @@ template <typename R> struct EngineImpl final : Engine {
         for (hipEvent_t e : ev_pool) (void) hipEventDestroy(e);
         if (ev_fork) (void) hipEventDestroy(ev_fork);
         if (ev_join) (void) hipEventDestroy(ev_join);
+        if (s_att) (void) hipStreamDestroy(s_att);
     }
 
     int open(const EngineGraph *gr, int ngraphs) override;
@@ -2548,13 +2549,11 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         hs.check(hipMemsetAsync(d_F[1].p, 0, nn * sizeof(double2), stream), "memset");
     }
     if (hs.ok()) hs.check(hipMemsetAsync(d_Fa.p, 0, nn * sizeof(double2), stream), "memset");
-    // the attraction's stream (the device's side stream, shared by the
-    // batches of the device as the main stream is) and the events that fork
-    // it from the batch's stream and join it back (not used without overlap)
-    if (hs.ok() && overlap && !s_att) {
-        s_att = device_side_stream(device);
-        if (!s_att) hs.check(hipErrorInvalidValue, "stream");
-    }
+    // the attraction's stream (default priority) and the events that fork it
+    // from the batch's stream and join it back (not made without overlap:
+    // an idle stream still takes a share of the hardware queues)
+    if (hs.ok() && overlap && !s_att)
+        hs.check(hipStreamCreateWithFlags(&s_att, hipStreamNonBlocking), "stream");
     if (hs.ok() && !ev_fork) hs.check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "event");
     if (hs.ok() && !ev_join) hs.check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "event");
     if (hs.ok()) {
