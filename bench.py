@@ -56,18 +56,14 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# The serving leg runs 8 streams at once. HIP gives a process 4 hardware
-# queues by default, so pairs of streams would share one, and a stream's
-# copies (a batch being rebuilt) would wait behind the kernels the other
-# stream queued. Read by the HIP runtime when it starts, before any GPU call.
-# A caller's value of at least 16 is kept; a smaller one (environments often
-# export HIP's default of 4) is raised to 16.
-try:
-    _hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
-except ValueError:
-    _hwq = 0
-if _hwq < 16:
-    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+# The serving leg runs 8 streams at once on the hardware queues HIP gives the
+# process (GPU_MAX_HW_QUEUES, 4 by default), which the bench leaves as the
+# environment sets them. With 16 queues the server's batch rebuilds waited
+# less (lane time 12 -> 5.6 s on the 200-request stream), but the served rate
+# did not rise (17.2 against 17.6 layouts/s), and a layout made after the
+# server or after a repulsion-offload session in the same process ran up to
+# 2.7 times slower: the attraction's stream, forked beside the traversal,
+# then shared a hardware queue with the batch's own.
 
 METRIC = ("FA2 node-iters/s at 2M nodes; served layouts/s + p99 vs "
           "one-at-a-time plain")
