@@ -141,38 +141,6 @@ k_tile_bbox(const Tile *__restrict__ tiles, const typename Vec<R>::v2 *__restric
     if (threadIdx.x == 0) tbox[blockIdx.x] = mk4<R>(a, b, c, d);
 }
 
-// Bounding box of each graph from its tiles: one wave per graph.
-template <typename R>
-__global__ void __launch_bounds__(256)
-k_graph_bbox(int G, const GraphDesc *__restrict__ gd, const GraphState *__restrict__ gs,
-             const typename Vec<R>::v4 *__restrict__ tbox, typename Vec<R>::v4 *__restrict__ gbox)
-{
-    const int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (g >= G || gs[g].retired) return;
-    const int lane = lane_id();
-    const int t0 = gd[g].toff, nt = gd[g].ntiles;
-    R a = (R) INFINITY, b = (R) INFINITY, c = (R) -INFINITY, d = (R) -INFINITY;
-    // eight boxes per lane in flight at once (a slot past the end reads the
-    // last box again, which changes no minimum or maximum)
-    for (int t = lane; t < nt; t += 64 * 8) {
-        typename Vec<R>::v4 v[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) v[u] = tbox[t0 + min(t + 64 * u, nt - 1)];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            a = min(a, v[u].x);
-            b = min(b, v[u].y);
-            c = max(c, v[u].z);
-            d = max(d, v[u].w);
-        }
-    }
-    a = wave_min(a);
-    b = wave_min(b);
-    c = wave_max(c);
-    d = wave_max(d);
-    if (lane == 0) gbox[g] = mk4<R>(a, b, c, d);
-}
-
 __device__ inline uint32_t spread16(uint32_t v)
 {
     v &= 0x0000FFFFu;
@@ -186,36 +154,85 @@ __device__ inline uint32_t spread16(uint32_t v)
 __device__ inline float div_rn(float a, float b) { return __fdiv_rn(a, b); }
 __device__ inline double div_rn(double a, double b) { return __ddiv_rn(a, b); }
 
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+static_assert(kSortThreads == 256, "the sort kernels hold one digit per thread, four waves");
+constexpr int kTile = kSortThreads * kSortItems; // 2048 nodes
+
 // Morton code of every node: 16 bits per axis, x in the even bits, computed
 // with IEEE arithmetic as written (the codes must match the reference bit for
 // bit, pipeline.cl:116-130).
 //
-// The tile's counts of the sort's first digit (the low 8 bits) are made here
-// too, as k_sort_upsweep makes them for the later passes, so the first pass
-// starts at its row scans.
+// The graph's bounding box (pipeline.cl:61-99) comes first: every workgroup
+// of the graph reduces the boxes of its tiles, which the last displacement
+// (or k_tile_bbox) made, itself (a minimum and a maximum are the same in
+// any order), while its own positions load. The tile's counts of the sort's
+// first digit (the low 8 bits) are made here too, as k_sort_upsweep makes
+// them for the later passes, so the first pass starts at its row scans.
 template <typename R>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kSortThreads)
 k_morton(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
          const GraphState *__restrict__ gs, const typename Vec<R>::v2 *__restrict__ pos,
-         const typename Vec<R>::v4 *__restrict__ gbox, uint32_t *__restrict__ key,
+         const typename Vec<R>::v4 *__restrict__ tbox, uint32_t *__restrict__ key,
          uint32_t *__restrict__ val, uint32_t *__restrict__ hist, int UT, int *__restrict__ ucnt)
 {
-    __shared__ uint32_t h[256]; // one digit's count per thread (256 threads)
+    using V2 = typename Vec<R>::v2;
+    using V4 = typename Vec<R>::v4;
+    __shared__ uint32_t h[kSortThreads]; // one digit's count per thread
+    __shared__ R s_box[4][4];            // each wave's part of the graph's box
     // the lengths of the summary unit lists, filled by k_karras
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < UT; q += gridDim.x * blockDim.x)
         ucnt[q * kCountStride] = 0;
     const Tile t = tiles[blockIdx.x];
     if (gs[t.g].retired) return; // its tree is not used any more
-    h[threadIdx.x] = 0;
-    const typename Vec<R>::v4 bb = gbox[t.g];
+    const GraphDesc g = gd[t.g];
+    const int tid = (int) threadIdx.x;
+    h[tid] = 0;
+    // the thread's positions (a slot past the tile's end reads its last
+    // position and takes no part)
+    V2 pv[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) pv[r] = pos[min(t.lo + tid + r * kSortThreads, t.hi - 1)];
+    R a = (R) INFINITY, b = (R) INFINITY, c = (R) -INFINITY, d = (R) -INFINITY;
+    // four tile boxes per thread in flight at once (a slot past the end
+    // reads the last box again, which changes no minimum or maximum)
+    for (int j = tid; j < g.ntiles; j += 4 * kSortThreads) {
+        V4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = tbox[g.toff + min(j + u * kSortThreads, g.ntiles - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a = min(a, v[u].x);
+            b = min(b, v[u].y);
+            c = max(c, v[u].z);
+            d = max(d, v[u].w);
+        }
+    }
+    a = wave_min(a);
+    b = wave_min(b);
+    c = wave_max(c);
+    d = wave_max(d);
+    const int w = tid >> 6;
+    if (lane_id() == 0) {
+        s_box[w][0] = a;
+        s_box[w][1] = b;
+        s_box[w][2] = c;
+        s_box[w][3] = d;
+    }
+    __syncthreads();
+    const V4 bb = mk4<R>(min(min(s_box[0][0], s_box[1][0]), min(s_box[2][0], s_box[3][0])),
+                         min(min(s_box[0][1], s_box[1][1]), min(s_box[2][1], s_box[3][1])),
+                         max(max(s_box[0][2], s_box[1][2]), max(s_box[2][2], s_box[3][2])),
+                         max(max(s_box[0][3], s_box[1][3]), max(s_box[2][3], s_box[3][3])));
     const R ex = bb.z - bb.x, ey = bb.w - bb.y;
     const R extent = ex > ey ? ex : ey;
     const R scale = extent > (R) 0 ? div_rn((R) 65535, extent) : (R) 0;
-    __syncthreads();
-    for (int i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
-        const typename Vec<R>::v2 p = pos[i];
-        R fx = (p.x - bb.x) * scale;
-        R fy = (p.y - bb.y) * scale;
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const int i = t.lo + tid + r * kSortThreads;
+        if (i >= t.hi) break;
+        R fx = (pv[r].x - bb.x) * scale;
+        R fy = (pv[r].y - bb.y) * scale;
         fx = min(max(fx, (R) 0), (R) 65535);
         fy = min(max(fy, (R) 0), (R) 65535);
         const uint32_t k = spread16((uint32_t) fx) | (spread16((uint32_t) fy) << 1);
@@ -224,8 +241,7 @@ k_morton(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
         atomicAdd(&h[k & 255u], 1u);
     }
     __syncthreads();
-    const GraphDesc g = gd[t.g];
-    hist[(size_t) g.toff * 256 + (size_t) threadIdx.x * g.ntiles + t.j] = h[threadIdx.x];
+    hist[(size_t) g.toff * 256 + (size_t) tid * g.ntiles + t.j] = h[tid];
 }
 
 // ---------------------------------------------------------------------------
@@ -234,11 +250,6 @@ k_morton(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
 // so every pass keeps each graph in its own node range.
 // hist row (g, d) starts at toff_g*256 + d*ntiles_g.
 // ---------------------------------------------------------------------------
-
-constexpr int kSortThreads = 256;
-constexpr int kSortItems = 8;
-static_assert(kSortThreads == 256, "the sort kernels hold one digit per thread, four waves");
-constexpr int kTile = kSortThreads * kSortItems; // 2048 nodes
 
 __global__ void __launch_bounds__(kSortThreads)
 k_sort_upsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
@@ -2014,7 +2025,7 @@ template <typename R> struct EngineImpl final : Engine {
     DevBuf<R> d_sell_coef;
     DevBuf<int4> d_items, d_hubs;
     DevBuf<double2> d_hubpart;
-    DevBuf<V4> d_tbox, d_gbox, d_psort, d_nd;
+    DevBuf<V4> d_tbox, d_psort, d_nd;
     DevBuf<uint32_t> d_keyA, d_valA, d_keyB, d_valB, d_hist, d_rowtot;
     DevBuf<int2> d_child;
     DevBuf<TNode<R>> d_tn;
@@ -2080,8 +2091,9 @@ template <typename R> struct EngineImpl final : Engine {
     void enqueue_iteration(std::vector<hipEvent_t> *ev);
     void mark(std::vector<hipEvent_t> *ev);
     // phases 1-4: codes, sort, tree of the current positions (scratch
-    // buffers only); `fork` (0, 1, 2) starts the attraction on the second
-    // stream before the codes, after the sort or after the tree, -1 never
+    // buffers only); `fork` (0, 1) starts the attraction on the second
+    // stream before the codes or after the sort, -1 never (enqueue_forces
+    // forks it after the tree)
     void enqueue_tree(std::vector<hipEvent_t> *ev, int fork = -1);
     // phases 1-6 without profiling events: the attraction forked at `fork`
     // (0..2) on the second stream and joined after the traversal, or (3)
@@ -2462,7 +2474,6 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     hs.check(d_hubs.alloc(h_hubs.size(), ast), "alloc");
     hs.check(d_hubpart.alloc(nslots, ast), "alloc");
     hs.check(d_tbox.alloc(T, ast), "alloc");
-    hs.check(d_gbox.alloc(G, ast), "alloc");
     hs.check(d_psort.alloc(nn + kLeafSize, ast), "alloc");
     hs.check(d_nd.alloc(2 * (size_t) (NI > 0 ? NI : 1), ast), "alloc");
     hs.check(d_keyA.alloc(nn, ast), "alloc");
@@ -2637,9 +2648,19 @@ template <typename R> void EngineImpl<R>::fork_attraction()
 template <typename R> void EngineImpl<R>::enqueue_forces(double2 *Fout, int fork)
 {
     if (!s_att) fork = 3;
-    enqueue_tree(nullptr, fork < 3 ? fork : -1);
-    // 5: repulsion minus gravity
+    enqueue_tree(nullptr, fork < 2 ? fork : -1);
+    // 5: repulsion minus gravity. Beside the traversal (fork 2) the
+    // attraction's branch starts from the same point but is launched after
+    // the traversal: captured in this order, the traversal starts 6 us
+    // sooner after the tree in the replayed graph (C3: 1.032 -> 1.026 ms per
+    // iteration), and the attraction, which ends well before it, later.
+    if (fork == 2) (void) hipEventRecord(ev_fork, stream);
     launch_repulse<false>(Fout, nullptr);
+    if (fork == 2) {
+        (void) hipStreamWaitEvent(s_att, ev_fork, 0);
+        launch_attraction(s_att, true);
+        (void) hipEventRecord(ev_join, s_att);
+    }
     if (fork < 3) (void) hipStreamWaitEvent(stream, ev_join, 0);
     else launch_attraction(stream);
 }
@@ -2679,8 +2700,7 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
     if (fork == 0) fork_attraction();
     mark(ev);
     // 1-2: codes
-    k_graph_bbox<R><<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_gs.p, d_tbox.p, d_gbox.p);
-    k_morton<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, d_pos.p, d_gbox.p, d_keyA.p,
+    k_morton<R><<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, d_pos.p, d_tbox.p, d_keyA.p,
                                        d_valA.p, d_hist.p, (int) h_units.size(), d_ucnt.p);
     mark(ev);
     // 3: sort, A -> B -> A -> B -> A (k_karras gathers psort)
@@ -2720,7 +2740,6 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
                     d_tn.p);
     }
     mark(ev);
-    if (fork == 2) fork_attraction();
 }
 
 // Record two iterations into a HIP graph (no memory set or copy is part of
