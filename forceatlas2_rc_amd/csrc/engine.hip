@@ -661,7 +661,7 @@ template <typename R>
 __global__ void __launch_bounds__(kKarrasThreads)
 k_karras(int NI, int N, const int *__restrict__ gi, const GraphDesc *__restrict__ gd,
          const GraphState *__restrict__ gs, const uint32_t *__restrict__ ks,
-         int2 *__restrict__ child, TNode<R> *__restrict__ tn, int *__restrict__ parent_leaf,
+         int2 *__restrict__ child, int2 *__restrict__ rng, int *__restrict__ parent_leaf,
          int *__restrict__ parent_int, int *__restrict__ escape, const int *__restrict__ ubase,
          int *__restrict__ ucnt, int *__restrict__ lists, int *__restrict__ xslot,
          const uint32_t *__restrict__ order, const typename Vec<R>::v4 *__restrict__ node4,
@@ -733,7 +733,7 @@ k_karras(int NI, int N, const int *__restrict__ gi, const GraphDesc *__restrict_
     else parent_leaf[~right] = t;
     if (i == 0) parent_int[t] = -1;
     escape[S - g] = right;
-    tn[t].l = make_int4(record_of(left, NI), 0, base + first, base + last);
+    rng[t] = make_int2(base + first, base + last);
     // units are powers of two: level L splits positions at bit
     // log2(block) + 4 (L - 1); positions stay below 2^26, so the shift stays
     // below 32
@@ -817,8 +817,9 @@ __global__ void __launch_bounds__(1024)
 k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
                  const GraphState *__restrict__ gs,
                  const typename Vec<R>::v4 *__restrict__ psort, const int2 *__restrict__ child,
-                 const int *__restrict__ parent_leaf, const int *__restrict__ parent_int,
-                 const int *__restrict__ escape, int NI, typename Vec<R>::v4 *nd, TNode<R> *tn)
+                 const int2 *__restrict__ rng, const int *__restrict__ parent_leaf,
+                 const int *__restrict__ parent_int, const int *__restrict__ escape, int NI,
+                 typename Vec<R>::v4 *nd, TNode<R> *tn)
 {
     using V4 = typename Vec<R>::v4;
     constexpr int B = agg_block<R>();
@@ -853,16 +854,17 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     V4 leaf = psort[pos];
     int esc_pos = escape[min(pos, gend - 2) - g];
     int pl = parent_leaf[pos];
-    int4 l = make_int4(0, 0, 0, 0);
+    int2 rg = make_int2(0, 0); // the region's range of sorted positions
     int2 ch = make_int2(0, 0);
     int par = -1;
     if (nslots > 0) { // a block of the graph's last position alone has no region
-        l = tn[t].l;
+        rg = rng[t];
         ch = child[t];
         par = parent_int[t];
     }
     asm volatile("" : "+v"(leaf.x), "+v"(leaf.y), "+v"(leaf.z), "+v"(leaf.w), "+v"(esc_pos),
-                 "+v"(pl), "+v"(l.z), "+v"(l.w), "+v"(ch.x), "+v"(ch.y), "+v"(par));
+                 "+v"(pl), "+v"(rg.x), "+v"(rg.y), "+v"(ch.x), "+v"(ch.y), "+v"(par));
+    const int4 l = make_int4(record_of(ch.x, NI), 0, rg.x, rg.y);
     const bool inside = has_slot && l.z >= lo && l.w < hi;
     // the skip link of a region merged here, loaded now for the end
     const int esc_last = inside ? escape[min(l.w, gend - 2) - g] : 0;
@@ -968,7 +970,8 @@ template <typename R>
 __global__ void __launch_bounds__(1024)
 k_aggregate_unit(const Unit *__restrict__ units, int level, const GraphDesc *__restrict__ gd,
                  GraphState *__restrict__ gs, const typename Vec<R>::v4 *__restrict__ psort,
-                 const int2 *__restrict__ child, const int *__restrict__ parent_int,
+                 const int2 *__restrict__ child, const int2 *__restrict__ rng,
+                 const int *__restrict__ parent_int,
                  const int *__restrict__ escape, const int *__restrict__ xslot,
                  const int *__restrict__ lists, const int *__restrict__ ucnt, int NI,
                  typename Vec<R>::v4 *nd, TNode<R> *tn)
@@ -1008,7 +1011,7 @@ k_aggregate_unit(const Unit *__restrict__ units, int level, const GraphDesc *__r
         const int t = lists[(size_t) U.index * kUnitCap + q];
         const int2 c = child[t];
         const int p = parent_int[t];
-        const int last = tn[t].l.w;
+        const int last = rng[t].y;
         const int xl = c.x >= 0 ? xslot[c.x] : 0, xr = c.y >= 0 ? xslot[c.y] : 0;
         const int xp = p >= 0 ? xslot[p] : 0;
         const int esc = last == gend - 1 ? 0 : escape[last - U.g];
@@ -1065,8 +1068,8 @@ k_aggregate_unit(const Unit *__restrict__ units, int level, const GraphDesc *__r
         const Summ<R> r = get(q);
         // the whole record in one 32-byte write
         TNode<R> rec;
-        rec.l = tn[t].l;
-        rec.l.y = s_skip[q];
+        const int2 rg = rng[t];
+        rec.l = make_int4(record_of(child[t].x, NI), s_skip[q], rg.x, rg.y);
         rec.c = region_record<R>(r);
         tn[t] = rec;
         if (s_ps[q] == -1) { // merged by a higher unit
@@ -2027,7 +2030,7 @@ template <typename R> struct EngineImpl final : Engine {
     DevBuf<double2> d_hubpart;
     DevBuf<V4> d_tbox, d_psort, d_nd;
     DevBuf<uint32_t> d_keyA, d_valA, d_keyB, d_valB, d_hist, d_rowtot;
-    DevBuf<int2> d_child;
+    DevBuf<int2> d_child, d_rng;
     DevBuf<TNode<R>> d_tn;
     DevBuf<int> d_parent_leaf, d_parent_int, d_escape;
     // summary units of level >= 2, level after level (units of level L at
@@ -2484,6 +2487,7 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
     hs.check(d_rowtot.alloc(256 * (size_t) G, ast), "alloc");
     for (int c = 0; c < 3; c++) hs.check(d_rowg[c].alloc(h_rowg[c].size(), ast), "alloc");
     hs.check(d_child.alloc(NI, ast), "alloc");
+    hs.check(d_rng.alloc(NI, ast), "alloc");
     hs.check(d_tn.alloc((size_t) NI + nn, ast), "alloc"); // regions, then one record per leaf
     hs.check(d_parent_leaf.alloc(nn, ast), "alloc");
     hs.check(d_parent_int.alloc(NI, ast), "alloc");
@@ -2727,15 +2731,15 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
     // 4: tree
     if (NI > 0) {
         k_karras<R><<<ceil_div(NI, kKarrasThreads), kKarrasThreads, 0, stream>>>(
-            NI, N, d_gi.p, d_gd.p, d_gs.p, d_keyA.p, d_child.p, d_tn.p, d_parent_leaf.p, d_parent_int.p,
+            NI, N, d_gi.p, d_gd.p, d_gs.p, d_keyA.p, d_child.p, d_rng.p, d_parent_leaf.p, d_parent_int.p,
             d_escape.p, d_ubase.p, d_ucnt.p, d_lists.p, d_xslot.p, d_valA.p, d_node4.p, d_psort.p);
         k_aggregate_tile<R><<<T * (kTile / agg_block<R>()), 1024, 0, stream>>>(
-            d_tiles.p, d_gd.p, d_gs.p, d_psort.p, d_child.p, d_parent_leaf.p, d_parent_int.p,
-            d_escape.p, NI, d_nd.p, d_tn.p);
+            d_tiles.p, d_gd.p, d_gs.p, d_psort.p, d_child.p, d_rng.p, d_parent_leaf.p,
+            d_parent_int.p, d_escape.p, NI, d_nd.p, d_tn.p);
         for (int L = 2; L < kMaxLevels; L++)
             if (level_cnt[L] > 0)
                 k_aggregate_unit<R><<<level_cnt[L], 1024, 0, stream>>>(
-                    d_units.p + level_off[L], L, d_gd.p, d_gs.p, d_psort.p, d_child.p,
+                    d_units.p + level_off[L], L, d_gd.p, d_gs.p, d_psort.p, d_child.p, d_rng.p,
                     d_parent_int.p, d_escape.p, d_xslot.p, d_lists.p, d_ucnt.p, NI, d_nd.p,
                     d_tn.p);
     }
@@ -3466,7 +3470,7 @@ template <typename R> static size_t footprint_of(long long n, long long incidenc
                        3 * sizeof(R) /*mass, size*/ + 1 /*fixed*/ + sizeof(int) /*inc_off*/ +
                        sizeof(int4) /*class lists*/ + sizeof(V4) /*psort*/ +
                        4 * sizeof(uint32_t) /*keys, values*/ + sizeof(int) /*parent_leaf*/);
-    b += (size_t) ni * (sizeof(int) /*gi*/ + 2 * sizeof(V4) /*nd*/ + sizeof(int2) /*child*/ +
+    b += (size_t) ni * (sizeof(int) /*gi*/ + 2 * sizeof(V4) /*nd*/ + 2 * sizeof(int2) /*child, range*/ +
                         sizeof(TNode<R>) + 3 * sizeof(int) /*parent, escape, xslot*/);
     // summary units of level >= 2: at most n / (15 * block) plus one per level
     const long long units = n / (15LL * agg_block<R>()) + kMaxLevels;
