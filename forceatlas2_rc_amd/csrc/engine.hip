@@ -86,6 +86,15 @@ struct Tile {
     int g, lo, hi, j; // graph, node range [lo, hi), tile index in the graph
 };
 
+// Tile b of a launch. When the batch is a single graph of N1 nodes (N1 > 0),
+// its tiles are those of one range, computed here rather than loaded: the
+// kernel's first loads then wait on no other.
+__device__ inline Tile tile_of(const Tile *__restrict__ tiles, int b, int N1, int per_tile)
+{
+    if (N1 > 0) return Tile{0, b * per_tile, min(N1, (b + 1) * per_tile), b};
+    return tiles[b];
+}
+
 template <typename R> struct alignas(32) TNode {
     typename Vec<R>::v4 c; // centre x, centre y, mass, size^2
     int4 l;                // left child, skip, first, last (sorted positions)
@@ -174,7 +183,8 @@ __global__ void __launch_bounds__(kSortThreads)
 k_morton(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
          const GraphState *__restrict__ gs, const typename Vec<R>::v2 *__restrict__ pos,
          const typename Vec<R>::v4 *__restrict__ tbox, uint32_t *__restrict__ key,
-         uint32_t *__restrict__ val, uint32_t *__restrict__ hist, int UT, int *__restrict__ ucnt)
+         uint32_t *__restrict__ val, uint32_t *__restrict__ hist, int UT, int *__restrict__ ucnt,
+         int N1)
 {
     using V2 = typename Vec<R>::v2;
     using V4 = typename Vec<R>::v4;
@@ -183,7 +193,7 @@ k_morton(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
     // the lengths of the summary unit lists, filled by k_karras
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < UT; q += gridDim.x * blockDim.x)
         ucnt[q * kCountStride] = 0;
-    const Tile t = tiles[blockIdx.x];
+    const Tile t = tile_of(tiles, blockIdx.x, N1, kTile);
     if (gs[t.g].retired) return; // its tree is not used any more
     const GraphDesc g = gd[t.g];
     const int tid = (int) threadIdx.x;
@@ -254,10 +264,10 @@ k_morton(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
 __global__ void __launch_bounds__(kSortThreads)
 k_sort_upsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
                const GraphState *__restrict__ gs, const uint32_t *__restrict__ key, int shift,
-               uint32_t *__restrict__ hist)
+               uint32_t *__restrict__ hist, int N1)
 {
     __shared__ uint32_t h[256];
-    const Tile t = tiles[blockIdx.x];
+    const Tile t = tile_of(tiles, blockIdx.x, N1, kTile);
     if (gs[t.g].retired) return;
     h[threadIdx.x] = 0;
     // the thread's keys, every load in flight at once (a slot past the
@@ -395,7 +405,7 @@ k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
                  const GraphState *__restrict__ gs, const uint32_t *__restrict__ key_in,
                  const uint32_t *__restrict__ val_in,
                  int shift, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ rowtot,
-                 uint32_t *__restrict__ key_out, uint32_t *__restrict__ val_out)
+                 uint32_t *__restrict__ key_out, uint32_t *__restrict__ val_out, int N1)
 {
     __shared__ uint32_t wcnt[4][256];
     __shared__ int goff[256];
@@ -403,7 +413,7 @@ k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     __shared__ uint32_t sval[kTile];
     __shared__ uint32_t scan_tmp[256];
 
-    const Tile t = tiles[blockIdx.x];
+    const Tile t = tile_of(tiles, blockIdx.x, N1, kTile);
     if (gs[t.g].retired) return;
     const GraphDesc G = gd[t.g];
     const int cnt = t.hi - t.lo;
@@ -561,7 +571,7 @@ int radix_sort_pairs_u32(hipStream_t st, hipStream_t ast, int n, int bits, uint3
     int in = 0;
     for (int pass = 0; hs.ok() && pass < passes; pass++) {
         k_sort_upsweep<<<T, kSortThreads, 0, st>>>(d_tiles.p, d_gd.p, d_gs.p, k[in], 8 * pass,
-                                                    d_hist.p);
+                                                    d_hist.p, n);
         if (T <= kRowThreadTiles)
             k_sort_rowscan_thread<<<1, 256, 0, st>>>(256, d_glist.p, d_gd.p, d_gs.p, d_hist.p,
                                                      d_rowtot.p);
@@ -573,7 +583,7 @@ int radix_sort_pairs_u32(hipStream_t st, hipStream_t ast, int n, int bits, uint3
                                                       d_rowtot.p);
         k_sort_downsweep<<<T, kSortThreads, 0, st>>>(d_tiles.p, d_gd.p, d_gs.p, k[in], v[in],
                                                       8 * pass, d_hist.p, d_rowtot.p, k[in ^ 1],
-                                                      v[in ^ 1]);
+                                                      v[in ^ 1], n);
         hs.check(hipGetLastError(), "launch");
         in ^= 1;
     }
@@ -665,14 +675,15 @@ k_karras(int NI, int N, const int *__restrict__ gi, const GraphDesc *__restrict_
          int *__restrict__ parent_int, int *__restrict__ escape, const int *__restrict__ ubase,
          int *__restrict__ ucnt, int *__restrict__ lists, int *__restrict__ xslot,
          const uint32_t *__restrict__ order, const typename Vec<R>::v4 *__restrict__ node4,
-         typename Vec<R>::v4 *__restrict__ psort)
+         typename Vec<R>::v4 *__restrict__ psort, int N1)
 {
     __shared__ uint32_t s_key[kKarrasStage];
     const int t0 = (int) blockIdx.x * kKarrasThreads;
     const int t = t0 + (int) threadIdx.x;
     const int tl = min(t0 + kKarrasThreads - 1, NI - 1);
     // the region's graph and descriptor are loaded beside the keys
-    const int g = gi[min(t, NI - 1)];
+    // (a batch of one graph: graph 0, whose regions sit at their own positions)
+    const int g = N1 > 0 ? 0 : gi[min(t, NI - 1)];
     const bool retired = gs[g].retired;
     const GraphDesc G = gd[g];
     // The gather of the node records in sorted order (pipeline.cl:195-207):
@@ -682,8 +693,8 @@ k_karras(int NI, int N, const int *__restrict__ gi, const GraphDesc *__restrict_
     // they wait beside the key staging and the searches.
     const int pg = min(t, NI - 1) + g;
     const typename Vec<R>::v4 nv = node4[order[pg]];
-    const int lo = t0 + gi[t0] - kKarrasWin; // s_key[q] holds the key at position lo + q
-    const int cnt = min(tl + gi[tl] + kKarrasWin + 1 - lo, kKarrasStage);
+    const int lo = t0 + (N1 > 0 ? 0 : gi[t0]) - kKarrasWin; // s_key[q]: the key at position lo + q
+    const int cnt = min(tl + (N1 > 0 ? 0 : gi[tl]) + kKarrasWin + 1 - lo, kKarrasStage);
     constexpr int kPer = kKarrasStage / kKarrasThreads;
     uint32_t kv[kPer];
 #pragma unroll
@@ -819,7 +830,7 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
                  const typename Vec<R>::v4 *__restrict__ psort, const int2 *__restrict__ child,
                  const int2 *__restrict__ rng, const int *__restrict__ parent_leaf,
                  const int *__restrict__ parent_int, const int *__restrict__ escape, int NI,
-                 typename Vec<R>::v4 *nd, TNode<R> *tn)
+                 typename Vec<R>::v4 *nd, TNode<R> *tn, int N1)
 {
     using V4 = typename Vec<R>::v4;
     constexpr int B = agg_block<R>();
@@ -830,7 +841,7 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     __shared__ int s_vis[B];
     __shared__ int s_span[B]; // last - first of each region inside
     __shared__ R s_sum[7][B]; // x0, y0, x1, y1, m, mx, my
-    const Tile T = tiles[blockIdx.x / kParts];
+    const Tile T = tile_of(tiles, blockIdx.x / kParts, N1, kTile);
     const int g = T.g;
     const int lo = T.lo + (int) (blockIdx.x % kParts) * B;
     const int hi = min(T.hi, lo + B);
@@ -1766,10 +1777,10 @@ k_swing(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
         const GraphState *__restrict__ gs, const R *__restrict__ mass,
         const uint8_t *__restrict__ fixed, double2 *__restrict__ F,
         const double2 *__restrict__ Fa, const double2 *__restrict__ Fold,
-        double2 *__restrict__ part)
+        double2 *__restrict__ part, int N1)
 {
     __shared__ double red[2][4];
-    const Tile t = tiles[blockIdx.x];
+    const Tile t = tile_of(tiles, blockIdx.x, N1, kTile);
     const bool use_fixed = gd[t.g].flags & GF_FIXED;
     // the force of the iteration: repulsion minus gravity, plus the
     // attraction (held apart while the two were computed side by side;
@@ -1890,10 +1901,10 @@ k_apply(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
         const uint8_t *__restrict__ fixed, const double2 *__restrict__ F,
         const double2 *__restrict__ Fold, typename Vec<R>::v2 *__restrict__ pos,
         const R *__restrict__ size, typename Vec<R>::v4 *__restrict__ node4,
-        double2 *__restrict__ acc, typename Vec<R>::v4 *__restrict__ tbox)
+        double2 *__restrict__ acc, typename Vec<R>::v4 *__restrict__ tbox, int N1)
 {
     __shared__ R lds[16];
-    const Tile t = tiles[blockIdx.x];
+    const Tile t = tile_of(tiles, blockIdx.x, N1, kTile);
     const GraphDesc D = gd[t.g];
     const GraphState S = gs[t.g];
     const bool move = S.active;
@@ -2000,6 +2011,9 @@ template <typename R> struct EngineImpl final : Engine {
     int nslots = 0;
     bool any_fp64_sums = false; // a graph with sizes or at theta 0: FP64 traversal sums
     int parity = 0; // F[parity] receives the force of the next iteration
+    // N when the batch is a single graph (its tiles and regions are then
+    // computed by the kernels rather than loaded, see tile_of), else 0
+    int N1 = 0;
 
     DevBuf<GraphDesc> d_gd;
     DevBuf<GraphState> d_gs;
@@ -2203,6 +2217,7 @@ template <typename R> int EngineImpl<R>::open(const EngineGraph *gr, int ngraphs
         toff += D.ntiles;
     }
     T = toff;
+    N1 = G == 1 ? N : 0;
     for (int g = 0; g < G; g++) {
         const int nt = h_gd[g].ntiles;
         h_rowg[nt <= kRowThreadTiles ? 0 : (nt <= 64 ? 1 : 2)].push_back(g);
@@ -2691,10 +2706,10 @@ template <typename R> void EngineImpl<R>::enqueue_iteration(std::vector<hipEvent
     // 7-8: the force (k_swing adds the attraction), swinging, speed,
     // displacement
     k_swing<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, d_mass.p, d_fixed.p, Fc, d_Fa.p,
-                                      Fo, d_part.p);
+                                      Fo, d_part.p, N1);
     k_speed<<<ceil_div(G, 4), 256, 0, stream>>>(G, d_gd.p, d_part.p, d_gs.p);
     k_apply<R><<<T, 256, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, d_mass.p, d_fixed.p, Fc, Fo,
-                                      d_pos.p, d_size.p, d_node4.p, d_acc.p, d_tbox.p);
+                                      d_pos.p, d_size.p, d_node4.p, d_acc.p, d_tbox.p, N1);
     mark(ev);
     parity = 1 - parity;
 }
@@ -2705,7 +2720,7 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
     mark(ev);
     // 1-2: codes
     k_morton<R><<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, d_pos.p, d_tbox.p, d_keyA.p,
-                                       d_valA.p, d_hist.p, (int) h_units.size(), d_ucnt.p);
+                                       d_valA.p, d_hist.p, (int) h_units.size(), d_ucnt.p, N1);
     mark(ev);
     // 3: sort, A -> B -> A -> B -> A (k_karras gathers psort)
     for (int pass = 0; pass < 4; pass++) {
@@ -2713,7 +2728,7 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
         uint32_t *ko = pass % 2 ? d_keyA.p : d_keyB.p, *vo = pass % 2 ? d_valA.p : d_valB.p;
         if (pass > 0) // the first pass's counts come from k_morton
             k_sort_upsweep<<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, ki, 8 * pass,
-                                                            d_hist.p);
+                                                            d_hist.p, N1);
         if (const int n0 = (int) h_rowg[0].size())
             k_sort_rowscan_thread<<<n0, 256, 0, stream>>>(256 * n0, d_rowg[0].p, d_gd.p, d_gs.p,
                                                           d_hist.p, d_rowtot.p);
@@ -2724,7 +2739,7 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
             k_sort_rowscan_block<<<256 * n2, 256, 0, stream>>>(d_rowg[2].p, d_gd.p, d_gs.p,
                                                                d_hist.p, d_rowtot.p);
         k_sort_downsweep<<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, ki, vi, 8 * pass,
-                                                          d_hist.p, d_rowtot.p, ko, vo);
+                                                          d_hist.p, d_rowtot.p, ko, vo, N1);
     }
     mark(ev);
     if (fork == 1) fork_attraction();
@@ -2732,10 +2747,11 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
     if (NI > 0) {
         k_karras<R><<<ceil_div(NI, kKarrasThreads), kKarrasThreads, 0, stream>>>(
             NI, N, d_gi.p, d_gd.p, d_gs.p, d_keyA.p, d_child.p, d_rng.p, d_parent_leaf.p, d_parent_int.p,
-            d_escape.p, d_ubase.p, d_ucnt.p, d_lists.p, d_xslot.p, d_valA.p, d_node4.p, d_psort.p);
+            d_escape.p, d_ubase.p, d_ucnt.p, d_lists.p, d_xslot.p, d_valA.p, d_node4.p, d_psort.p,
+            N1);
         k_aggregate_tile<R><<<T * (kTile / agg_block<R>()), 1024, 0, stream>>>(
             d_tiles.p, d_gd.p, d_gs.p, d_psort.p, d_child.p, d_rng.p, d_parent_leaf.p,
-            d_parent_int.p, d_escape.p, NI, d_nd.p, d_tn.p);
+            d_parent_int.p, d_escape.p, NI, d_nd.p, d_tn.p, N1);
         for (int L = 2; L < kMaxLevels; L++)
             if (level_cnt[L] > 0)
                 k_aggregate_unit<R><<<level_cnt[L], 1024, 0, stream>>>(
