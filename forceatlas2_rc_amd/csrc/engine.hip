@@ -399,7 +399,7 @@ k_sort_rowscan_block(const int *__restrict__ glist, const GraphDesc *__restrict_
 // the rank of a key counts the equal digits before it in the tile. Keys go
 // through LDS so that the global writes of one digit are contiguous. (The
 // node records in sorted order, pipeline.cl:195-207, are gathered by
-// k_karras, whose searches hide the random reads.)
+// k_aggregate_tile, beside its other first loads.)
 __global__ void __launch_bounds__(kSortThreads)
 k_sort_downsweep(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
                  const GraphState *__restrict__ gs, const uint32_t *__restrict__ key_in,
@@ -673,9 +673,7 @@ k_karras(int NI, int N, const int *__restrict__ gi, const GraphDesc *__restrict_
          const GraphState *__restrict__ gs, const uint32_t *__restrict__ ks,
          int2 *__restrict__ child, int2 *__restrict__ rng, int *__restrict__ parent_leaf,
          int *__restrict__ parent_int, int *__restrict__ escape, const int *__restrict__ ubase,
-         int *__restrict__ ucnt, int *__restrict__ lists, int *__restrict__ xslot,
-         const uint32_t *__restrict__ order, const typename Vec<R>::v4 *__restrict__ node4,
-         typename Vec<R>::v4 *__restrict__ psort, int N1)
+         int *__restrict__ ucnt, int *__restrict__ lists, int *__restrict__ xslot, int N1)
 {
     __shared__ uint32_t s_key[kKarrasStage];
     const int t0 = (int) blockIdx.x * kKarrasThreads;
@@ -686,13 +684,6 @@ k_karras(int NI, int N, const int *__restrict__ gi, const GraphDesc *__restrict_
     const int g = N1 > 0 ? 0 : gi[min(t, NI - 1)];
     const bool retired = gs[g].retired;
     const GraphDesc G = gd[g];
-    // The gather of the node records in sorted order (pipeline.cl:195-207):
-    // psort[p] = node4[order[p]] at the position p = t + g of region t (the
-    // graph's last position, which has no region, by its last region). The
-    // random reads are issued first and their records stored last, so that
-    // they wait beside the key staging and the searches.
-    const int pg = min(t, NI - 1) + g;
-    const typename Vec<R>::v4 nv = node4[order[pg]];
     const int lo = t0 + (N1 > 0 ? 0 : gi[t0]) - kKarrasWin; // s_key[q]: the key at position lo + q
     const int cnt = min(tl + (N1 > 0 ? 0 : gi[tl]) + kKarrasWin + 1 - lo, kKarrasStage);
     constexpr int kPer = kKarrasStage / kKarrasThreads;
@@ -765,8 +756,6 @@ k_karras(int NI, int N, const int *__restrict__ gi, const GraphDesc *__restrict_
         }
     }
     xslot[t] = xs;
-    psort[pg] = nv;
-    if (i == n - 2) psort[base + n - 1] = node4[order[base + n - 1]];
 }
 
 template <typename R> struct Summ {
@@ -826,8 +815,9 @@ constexpr int kOutside = 1 << 24;
 template <typename R>
 __global__ void __launch_bounds__(1024)
 k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ gd,
-                 const GraphState *__restrict__ gs,
-                 const typename Vec<R>::v4 *__restrict__ psort, const int2 *__restrict__ child,
+                 const GraphState *__restrict__ gs, const uint32_t *__restrict__ order,
+                 const typename Vec<R>::v4 *__restrict__ node4,
+                 typename Vec<R>::v4 *__restrict__ psort, const int2 *__restrict__ child,
                  const int2 *__restrict__ rng, const int *__restrict__ parent_leaf,
                  const int *__restrict__ parent_int, const int *__restrict__ escape, int NI,
                  typename Vec<R>::v4 *nd, TNode<R> *tn, int N1)
@@ -855,14 +845,16 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     const bool adjust = D.flags & GF_ADJUST;
     // One position per thread (a block has at most blockDim.x positions).
     // Every load of the thread is issued before the first wait: its leaf,
-    // the escape link after it, the links of region tbase + q and the leaf's
-    // parent; indices are clamped to valid entries and unused values
-    // dropped.
+    // gathered from the node records in their original order (pipeline.cl:
+    // 195-207: psort[p] = node4[order[p]], stored below for the traversal
+    // and the unit levels), the escape link after it, the links of region
+    // tbase + q and the leaf's parent; indices are clamped to valid entries
+    // and unused values dropped.
     const int q = (int) threadIdx.x;
     const bool has_leaf = q < hi - lo, has_slot = q < nslots;
     const int pos = lo + min(q, hi - lo - 1);
     const int t = tbase + min(q, max(nslots - 1, 0));
-    V4 leaf = psort[pos];
+    V4 leaf = node4[order[pos]];
     int esc_pos = escape[min(pos, gend - 2) - g];
     int pl = parent_leaf[pos];
     int2 rg = make_int2(0, 0); // the region's range of sorted positions
@@ -880,6 +872,7 @@ k_aggregate_tile(const Tile *__restrict__ tiles, const GraphDesc *__restrict__ g
     // the skip link of a region merged here, loaded now for the end
     const int esc_last = inside ? escape[min(l.w, gend - 2) - g] : 0;
     if (has_leaf) {
+        psort[pos] = leaf;
         s_leaf[q] = leaf;
         if (has_slot) {
             s_child[q] = ch;
@@ -2722,7 +2715,7 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
     k_morton<R><<<T, kSortThreads, 0, stream>>>(d_tiles.p, d_gd.p, d_gs.p, d_pos.p, d_tbox.p, d_keyA.p,
                                        d_valA.p, d_hist.p, (int) h_units.size(), d_ucnt.p, N1);
     mark(ev);
-    // 3: sort, A -> B -> A -> B -> A (k_karras gathers psort)
+    // 3: sort, A -> B -> A -> B -> A (k_aggregate_tile gathers psort)
     for (int pass = 0; pass < 4; pass++) {
         uint32_t *ki = pass % 2 ? d_keyB.p : d_keyA.p, *vi = pass % 2 ? d_valB.p : d_valA.p;
         uint32_t *ko = pass % 2 ? d_keyA.p : d_keyB.p, *vo = pass % 2 ? d_valA.p : d_valB.p;
@@ -2747,11 +2740,10 @@ template <typename R> void EngineImpl<R>::enqueue_tree(std::vector<hipEvent_t> *
     if (NI > 0) {
         k_karras<R><<<ceil_div(NI, kKarrasThreads), kKarrasThreads, 0, stream>>>(
             NI, N, d_gi.p, d_gd.p, d_gs.p, d_keyA.p, d_child.p, d_rng.p, d_parent_leaf.p, d_parent_int.p,
-            d_escape.p, d_ubase.p, d_ucnt.p, d_lists.p, d_xslot.p, d_valA.p, d_node4.p, d_psort.p,
-            N1);
+            d_escape.p, d_ubase.p, d_ucnt.p, d_lists.p, d_xslot.p, N1);
         k_aggregate_tile<R><<<T * (kTile / agg_block<R>()), 1024, 0, stream>>>(
-            d_tiles.p, d_gd.p, d_gs.p, d_psort.p, d_child.p, d_rng.p, d_parent_leaf.p,
-            d_parent_int.p, d_escape.p, NI, d_nd.p, d_tn.p, N1);
+            d_tiles.p, d_gd.p, d_gs.p, d_valA.p, d_node4.p, d_psort.p, d_child.p, d_rng.p,
+            d_parent_leaf.p, d_parent_int.p, d_escape.p, NI, d_nd.p, d_tn.p, N1);
         for (int L = 2; L < kMaxLevels; L++)
             if (level_cnt[L] > 0)
                 k_aggregate_unit<R><<<level_cnt[L], 1024, 0, stream>>>(
