@@ -619,8 +619,11 @@ def main():
     iter_roof = iteration_roofline(n, n_inc, flops, stages, ms * 1e-3, peak)
     log("iteration roofline:", json.dumps(iter_roof))
     traffic = None
+    # the counter-measured bytes of the single-precision traversal at this
+    # workload's default size (none are recorded for other runs)
     tpath = os.path.join(ROOT, "profiles", "repulse_rmat21_traffic.json")
-    if os.path.exists(tpath):
+    if (args.precision == "single" and args.scale == 21 and args.edges == 16_000_000
+            and os.path.exists(tpath)):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
     roof = dict(kernel="k_repulse (Barnes-Hut traversal)", bound="valu",
